@@ -1,0 +1,147 @@
+/*
+ * pyhea_hcs.h -- C ABI of the MI355X (gfx950) SRO-targeted lattice search.
+ *
+ * This is the drop-in boundary for PyHEA's native engine.  The reference
+ * binds its engine to Python with pybind11 (pyhea/cpp/bind.cpp:6-122,
+ * declarations pyhea/cpp/accelerate.h:23-66): run_local_parallel_hcs,
+ * run_local_parallel_hcs_cuda and cuda_available.  Those three names are kept
+ * by the pybind11 module pyhea_amd/cpp/accelerate, which is a thin shim over
+ * the functions below; any other FFI (ctypes, cffi) binds them directly.
+ *
+ * Conventions
+ *   - plain C types only; all buffers are caller-owned and nothing is retained
+ *     after a call returns unless a handle says so;
+ *   - every function returns HEA_OK (0) or a nonzero HEA_E* code and never
+ *     exits the process; hea_last_error() gives the message (thread-local);
+ *   - lattices are uint8 species indices, [solutions][n_sites] row-major;
+ *     fitness values are FP64 and bit-identical to the reference CPU engine
+ *     (pyhea/cpp/accelerate.cpp) driven by the same keyed streams;
+ *   - calls on one handle are not reentrant; distinct handles are independent.
+ */
+#ifndef PYHEA_HCS_H
+#define PYHEA_HCS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEA_OK 0
+#define HEA_EINVAL 1       /* invalid argument                              */
+#define HEA_ENODEV 2       /* no usable GPU                                 */
+#define HEA_EHIP 3         /* HIP runtime error                             */
+#define HEA_EUNSUPPORTED 4 /* valid input outside what the kernels support  */
+#define HEA_ENOMEM 5       /* device allocation failed                      */
+
+/* Stream phases: which draw of the search a keyed stream belongs to. */
+#define HEA_PHASE_INIT 0 /* initial population (accelerate.cpp:737)       */
+#define HEA_PHASE_NEW 1  /* per-iteration new population (:743)           */
+#define HEA_PHASE_LS1 2  /* local search I proposals (:745)               */
+#define HEA_PHASE_LS2 3  /* local search II proposals (:749)              */
+
+/* A search problem.  Replaces the reference's nested-vector arguments
+ * `neighbor_list`, `species`, `weight`, `target_sro` of
+ * accelerate::cpu::run_local_parallel_hcs (pyhea/cpp/accelerate.h:23-32). */
+typedef struct hea_problem_desc {
+  int32_t n_sites;      /* N                                                  */
+  int32_t n_species;    /* K, 2..8                                            */
+  int32_t n_shells;     /* S, 1..3 (len(weight), pyhea/model/opt.py:49-52)    */
+  int32_t shell_size[3];/* Z_s, 1..31; every site has Z_s members in shell s  */
+  const int32_t* nbr;   /* [N][sum_s Z_s]: shell s in columns
+                           [sum_{t<s} Z_t, +Z_s), each list ascending as
+                           pyhea/lattice/lattice.py:236-320 builds it          */
+  const int32_t* counts;/* [K] atoms per species, each >= 1, sum = N          */
+  const double* weight; /* [S] shell weights, finite and >= 0                 */
+  const double* target; /* [S][K*K] target Warren-Cowley alpha, row-major     */
+} hea_problem_desc;
+
+typedef struct hea_problem_s* hea_problem;
+typedef struct hea_session_s* hea_session;
+
+/* Counters of the work done; one swap eval = one incremental delta-F of a
+ * valid proposal. */
+typedef struct hea_stats {
+  int64_t evals;     /* swap evaluations                                   */
+  int64_t rounds;    /* local-search rounds                                */
+  int64_t accepts;   /* accepted swaps                                     */
+  int64_t draws;     /* proposal draws including same-species redraws     */
+  double first_hit_s;/* seconds from session start to the first fitness
+                        <= thr seen by local search, or -1 if never        */
+} hea_stats;
+
+/* Number of visible GPUs (0 if none or the runtime is unusable).
+ * Backs cuda_available() (pyhea/cpp/bind.cpp:90-120, pyhea/model/opt.py:87). */
+int hea_device_count(void);
+
+/* Message of the last failed call on this thread. */
+const char* hea_last_error(void);
+
+/* Validates a problem and makes it resident on `device`: neighbour table,
+ * per-problem constants and the exact R_s(k) segment tables.  The neighbour
+ * relation must be symmetric with multiplicity (true for every periodic
+ * lattice built by pyhea/lattice/lattice.py). */
+int hea_problem_create(const hea_problem_desc* desc, int device, hea_problem* out);
+int hea_problem_destroy(hea_problem p);
+
+/* Whole search: replaces accelerate::cpu::run_local_parallel_hcs and
+ * accelerate::gpu::run_local_parallel_hcs_cuda (pyhea/cpp/accelerate.h:23-32,
+ * 51-60; pyhea/cpp/accelerate.cpp:713-763).  Arguments keep the reference's
+ * meaning: solutions = nnet, total_iter = step, parallel_task = task
+ * (1..2048), converge_depth = depth, thr only sets the time-to-threshold
+ * statistic (the reference ignores it, accelerate.cpp:612, 718).  The final
+ * population is sorted by ascending fitness.  out_lattices [solutions][N],
+ * out_fitness [solutions], stats may be NULL. */
+int hea_hcs_run(hea_problem p, int32_t solutions, int32_t total_iter, int32_t parallel_task,
+                int32_t converge_depth, double thr, uint64_t seed, uint8_t* out_lattices,
+                double* out_fitness, hea_stats* stats);
+
+/* One local search over a caller-provided population (host buffers, updated
+ * in place), with streams (seed, iter, phase, slot = position).  Replaces
+ * accelerate::cpu::local_parallel_monte_carlo (accelerate.cpp:601-694);
+ * sort != 0 applies its closing sort by fitness (accelerate.cpp:693). */
+int hea_local_mc(hea_problem p, int32_t solutions, uint8_t* lattices, double* fitness,
+                 int32_t parallel_task, int32_t converge_depth, uint64_t seed, uint32_t iter,
+                 uint32_t phase, int32_t sort, hea_stats* stats);
+
+/* Same on device-resident buffers (d_lattices [solutions][N] uint8,
+ * d_fitness [solutions] double) on `stream` (a hipStream_t, NULL = default);
+ * asynchronous: no host synchronisation. */
+int hea_local_mc_device(hea_problem p, int32_t solutions, void* d_lattices, void* d_fitness,
+                        int32_t parallel_task, int32_t converge_depth, uint64_t seed,
+                        uint32_t iter, uint32_t phase, void* stream);
+
+/* Fitness by full recount: accelerate::cpu::calculate_fitness_of_solutions
+ * (accelerate.cpp:118-180). */
+int hea_fitness(hea_problem p, int32_t solutions, const uint8_t* lattices, double* out_fitness);
+
+/* Integer pair counts n[l][s][a][b] of each lattice (directed pairs). */
+int hea_pair_counts(hea_problem p, int32_t solutions, const uint8_t* lattices, int32_t* out);
+
+/* Keyed random lattices with their recount fitness: the role of
+ * accelerate::cpu::generate_random_lattices (accelerate.cpp:218-252). */
+int hea_generate(hea_problem p, int32_t solutions, uint64_t seed, uint32_t iter, uint32_t phase,
+                 uint8_t* out_lattices, double* out_fitness);
+
+/* Device-resident search session: the population stays in HBM and every
+ * global iteration (generate -> local search I -> merge -> local search II,
+ * accelerate.cpp:741-761) runs as one replayable HIP graph. */
+int hea_session_create(hea_problem p, int32_t solutions, int32_t parallel_task,
+                       int32_t converge_depth, double thr, uint64_t seed, hea_session* out);
+/* Initial population (accelerate.cpp:737); starts the session clock. */
+int hea_session_init(hea_session s);
+/* Runs `n_iter` global iterations asynchronously on the session stream. */
+int hea_session_step(hea_session s, int32_t n_iter);
+/* Waits for the session stream. */
+int hea_session_sync(hea_session s);
+/* Copies out the population (sorted) and the accumulated counters. */
+int hea_session_read(hea_session s, uint8_t* out_lattices, double* out_fitness, hea_stats* stats);
+/* Population best fitness after the last completed iteration. */
+int hea_session_best(hea_session s, double* best);
+int hea_session_destroy(hea_session s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYHEA_HCS_H */

@@ -1,0 +1,800 @@
+// Host side of the C ABI (include/pyhea_hcs.h) and the kernels around the
+// local search: keyed permutations, pair-count recount, recount fitness,
+// stable population merge, and the per-iteration HIP graph.
+//
+// Reference counterparts (pyhea/cpp/accelerate.cpp):
+//   generate_random_lattices     218-252  -> k_permute + k_recount + k_fitness
+//   calculate_fitness_of_solutions 118-180 -> k_recount + k_fitness
+//   sort_lattices_by_fitness       37-64  -> k_rank + k_gather
+//   calculate_best_lattices       305-330 -> k_rank (old ++ new) + k_gather
+//   run_local_parallel_hcs        713-763 -> hea_session_* (one graph/iteration)
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/pyhea_hcs.h"
+#include "hea_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HEA_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(HEA_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+// ------------------------------------------------------------------ kernels --
+
+// lattice[l][i] = sorted_multiset[perm_l(i)]
+__global__ void k_permute(const HeaConst* __restrict__ pc, uint8_t* __restrict__ out,
+                          uint64_t seed, const uint32_t* __restrict__ iter_ptr, uint32_t phase,
+                          int slot0) {
+  const HeaConst& P = *pc;
+  const int l = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.N) return;
+  const hea_perm_key key =
+      hea_perm_make_key(seed, *iter_ptr, phase, (uint32_t)(slot0 + l), (uint32_t)P.N);
+  const uint32_t q = hea_perm_apply(&key, (uint32_t)i);
+  out[(size_t)l * P.N + i] = (uint8_t)hea_species_of_rank(P.prefix, P.K, q);
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+constexpr int kRecountThreads = 256;
+constexpr int kRecountSites = 2048;  // sites per workgroup
+
+// Integer pair counts n[l][s][a][b]: per site, a packed histogram of the
+// neighbour species per shell, added into an LDS table, then into HBM.
+__global__ __launch_bounds__(kRecountThreads) void k_recount(
+    const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
+    const uint8_t* __restrict__ lattices, int32_t* __restrict__ counts) {
+  __shared__ int32_t hist[HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
+  const HeaConst& P = *pc;
+  const int l = blockIdx.y;
+  const int K = P.K, S = P.S;
+  for (int e = threadIdx.x; e < S * K * K; e += blockDim.x) hist[e] = 0;
+  __syncthreads();
+  const uint8_t* lat = lattices + (size_t)l * P.N;
+  const int i0 = blockIdx.x * kRecountSites;
+  const int i1 = min(P.N, i0 + kRecountSites);
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int a = lat[i];
+    const int32_t* row = nbr + (size_t)i * P.stride;
+    for (int s = 0; s < S; ++s) {
+      uint64_t m = 0;  // 8-bit fields, Z <= 31
+      for (int k = 0; k < P.Z[s]; ++k) m += 1ull << (8 * lat[row[P.off[s] + k]]);
+      for (int x = 0; x < K; ++x) {
+        const int f = (int)((m >> (8 * x)) & 255u);
+        if (f) atomicAdd(&hist[(s * K + a) * K + x], f);
+      }
+    }
+  }
+  __syncthreads();
+  int32_t* dst = counts + (size_t)l * S * K * K;
+  for (int e = threadIdx.x; e < S * K * K; e += blockDim.x)
+    if (hist[e]) atomicAdd(&dst[e], hist[e]);
+}
+
+// Fitness of freshly recounted lattices: gamma = R_s(n), then the
+// reference's sum (accelerate.cpp:165-178; std::pow(x, 2) there is x*x).
+__global__ void k_fitness(const HeaConst* __restrict__ pc, const int32_t* __restrict__ counts,
+                          double* __restrict__ fit, int L) {
+  const HeaConst& P = *pc;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= L) return;
+  const int K = P.K, S = P.S;
+  const int32_t* c = counts + (size_t)l * S * K * K;
+  double error = 0.0;
+  for (int s = 0; s < S; ++s) {
+    double es = 0.0;
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j < K; ++j) {
+        const double g = hea_r_of_count(P, s, c[(s * K + i) * K + j]);
+        const double sro = 1.0 - g / (P.conc[i] * P.conc[j]);
+        const double df = sro - P.target[s][i][j];
+        es = es + df * df;
+      }
+    error = error + P.weight[s] * es;
+  }
+  fit[l] = sqrt(error);
+}
+
+// Stable rank by (fitness, position) over fa[0..na) ++ fb[0..nb); writes
+// perm[rank] = position for ranks below `keep`.  Fitness values are finite
+// and >= 0, so comparing bit patterns as unsigned integers orders them.
+__global__ void k_rank(const double* __restrict__ fa, int na, const double* __restrict__ fb,
+                       int nb, int keep, int32_t* __restrict__ perm) {
+  const int n = na + nb;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double fi = i < na ? fa[i] : fb[i - na];
+  int r = 0;
+  for (int j = 0; j < n; ++j) {
+    const double fj = j < na ? fa[j] : fb[j - na];
+    r += (fj < fi) || (fj == fi && j < i);
+  }
+  if (r < keep) perm[r] = i;
+}
+
+// dst row r = (perm[r] < na ? srcA[perm[r]] : srcB[perm[r]-na]).
+__global__ void k_gather(const uint8_t* __restrict__ srcA, const double* __restrict__ fA, int na,
+                         const uint8_t* __restrict__ srcB, const double* __restrict__ fB,
+                         const int32_t* __restrict__ perm, uint8_t* __restrict__ dst,
+                         double* __restrict__ fdst, int N) {
+  const int r = blockIdx.y;
+  const int p = perm[r];
+  const uint8_t* src = p < na ? srcA + (size_t)p * N : srcB + (size_t)(p - na) * N;
+  uint8_t* d = dst + (size_t)r * N;
+  if ((N & 15) == 0) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(d);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N / 16; i += gridDim.x * blockDim.x)
+      d4[i] = s4[i];
+  } else {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+      d[i] = src[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) fdst[r] = p < na ? fA[p] : fB[p - na];
+}
+
+__global__ void k_iter_add(uint32_t* it, uint32_t v) { *it += v; }
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void k_clock(unsigned long long* t) { *t = wall_clock64(); }
+
+// ------------------------------------------------------------ host objects --
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (!count) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+}  // namespace
+
+struct hea_problem_s {
+  int device = 0;
+  HeaConst hc;
+  bool sp_lds = true;
+  int64_t max_lds = 0;
+  DevBuf<HeaConst> d_const;
+  DevBuf<int32_t> d_nbr;
+  // scratch for the host-buffer entry points (grown on demand)
+  DevBuf<uint8_t> d_lat, d_lat2;
+  DevBuf<double> d_fit, d_fit2;
+  DevBuf<int32_t> d_counts, d_perm;
+  DevBuf<int64_t> d_stats;
+  DevBuf<uint32_t> d_iter;
+};
+
+struct hea_session_s {
+  hea_problem p = nullptr;
+  int L = 0, task = 0, depth = 0;
+  double thr = 0.0;
+  uint64_t seed = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t lds = 0;
+  DevBuf<uint8_t> A, B, C;
+  DevBuf<double> fA, fB, fC;
+  DevBuf<int32_t> counts, perm;
+  DevBuf<int64_t> stats;
+  DevBuf<uint32_t> iter;
+  DevBuf<unsigned long long> clk;  // [0] start, [1] first fitness <= thr
+  int iters_done = 0;
+};
+
+namespace {
+
+int check_problem(const hea_problem_desc* d, std::string& why) {
+  if (!d) return why = "null problem", 0;
+  if (d->n_sites < 2 || d->n_sites > (1 << 26)) return why = "n_sites out of range [2, 2^26]", 0;
+  if (d->n_species < 2 || d->n_species > HEA_MAX_K)
+    return why = "n_species out of range [2, 8]", 0;
+  if (d->n_shells < 1 || d->n_shells > HEA_MAX_S) return why = "n_shells out of range [1, 3]", 0;
+  if (!d->nbr || !d->counts || !d->weight || !d->target) return why = "null array", 0;
+  for (int s = 0; s < d->n_shells; ++s)
+    if (d->shell_size[s] < 1 || d->shell_size[s] > HEA_MAX_Z)
+      return why = "shell_size out of range [1, 31]", 0;
+  int64_t tot = 0;
+  for (int a = 0; a < d->n_species; ++a) {
+    if (d->counts[a] < 1) return why = "every species needs at least one atom", 0;
+    tot += d->counts[a];
+  }
+  if (tot != d->n_sites) return why = "species counts do not sum to n_sites", 0;
+  for (int s = 0; s < d->n_shells; ++s)
+    if (!std::isfinite(d->weight[s]) || d->weight[s] < 0.0)
+      return why = "weights must be finite and >= 0", 0;
+  for (int e = 0; e < d->n_shells * d->n_species * d->n_species; ++e)
+    if (!std::isfinite(d->target[e])) return why = "target must be finite", 0;
+  return 1;
+}
+
+// Segments of R(k) = k-fold sequential sum of delta, verified at every k.
+int build_segments(double delta, int64_t kmax, int64_t* k0, double* v0, double* inc, int& nseg) {
+  nseg = 0;
+  double r = 0.0;        // R(k)
+  double rn = delta;     // R(k+1)
+  int64_t sk = 0;
+  double sv = 0.0, si = rn - r;
+  k0[0] = 0;
+  v0[0] = 0.0;
+  inc[0] = si;
+  nseg = 1;
+  for (int64_t k = 1; k <= kmax; ++k) {
+    r = rn;
+    rn = r + delta;
+    const double pred = std::fma((double)(k - sk), si, sv);
+    if (pred != r) {
+      if (nseg == HEA_MAX_SEG) return 0;
+      sk = k;
+      sv = r;
+      si = rn - r;
+      k0[nseg] = sk;
+      v0[nseg] = sv;
+      inc[nseg] = si;
+      ++nseg;
+    }
+  }
+  return 1;
+}
+
+int ensure_scratch(hea_problem p, int L) {
+  const HeaConst& c = p->hc;
+  const size_t lat = (size_t)L * c.N;
+  if (p->d_lat.n < lat) {
+    HEA_HIP(p->d_lat.alloc(lat));
+    HEA_HIP(p->d_lat2.alloc(lat));
+  }
+  if (p->d_fit.n < (size_t)L) {
+    HEA_HIP(p->d_fit.alloc(L));
+    HEA_HIP(p->d_fit2.alloc(L));
+    HEA_HIP(p->d_stats.alloc((size_t)4 * L));
+    HEA_HIP(p->d_perm.alloc((size_t)2 * L));
+  }
+  const size_t cn = (size_t)L * c.S * c.K * c.K;
+  if (p->d_counts.n < cn) HEA_HIP(p->d_counts.alloc(cn));
+  if (!p->d_iter.p) HEA_HIP(p->d_iter.alloc(1));
+  return HEA_OK;
+}
+
+int launch_recount(hea_problem p, const uint8_t* lat, int32_t* counts, int L, hipStream_t st) {
+  const HeaConst& c = p->hc;
+  const size_t cn = (size_t)L * c.S * c.K * c.K;
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)std::min<size_t>((cn + 255) / 256, 4096)),
+                     dim3(256), 0, st, counts, cn);
+  dim3 g((c.N + kRecountSites - 1) / kRecountSites, L);
+  hipLaunchKernelGGL(k_recount, g, dim3(kRecountThreads), 0, st, p->d_const.p, p->d_nbr.p, lat,
+                     counts);
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+int launch_fitness(hea_problem p, const int32_t* counts, double* fit, int L, hipStream_t st) {
+  hipLaunchKernelGGL(k_fitness, dim3((L + 63) / 64), dim3(64), 0, st, p->d_const.p, counts, fit,
+                     L);
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+// dst = stable order of (fa ++ fb), first `keep` rows.
+int launch_merge(const uint8_t* A, const double* fa, int na, const uint8_t* B, const double* fb,
+                 int nb, int keep, int32_t* perm, uint8_t* dst, double* fdst, int N,
+                 hipStream_t st) {
+  const int n = na + nb;
+  hipLaunchKernelGGL(k_rank, dim3((n + 255) / 256), dim3(256), 0, st, fa, na, fb, nb, keep, perm);
+  const int per = (N % 16 == 0) ? N / 16 : N;
+  const unsigned gx = (unsigned)std::max(1, std::min(64, (per + 255) / 256));
+  hipLaunchKernelGGL(k_gather, dim3(gx, keep), dim3(256), 0, st, A, fa, na, B, fb, perm, dst,
+                     fdst, N);
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, int64_t* stats,
+              const uint32_t* iter_ptr, unsigned long long* first_hit, int L, int task,
+              int depth, uint64_t seed, uint32_t phase, double thr, size_t lds, hipStream_t st) {
+  HeaMcArgs a;
+  a.pc = p->d_const.p;
+  a.nbr = p->d_nbr.p;
+  a.lattices = lat;
+  a.fitness = fit;
+  a.counts = counts;
+  a.stats = stats;
+  a.iter_ptr = iter_ptr;
+  a.first_hit = first_hit;
+  a.seed = seed;
+  a.L = L;
+  a.task = task;
+  a.depth = depth;
+  a.phase = phase;
+  a.thr = thr;
+  HEA_HIP(hea_launch_local_mc(a, p->hc.K, p->sp_lds, lds, st));
+  return HEA_OK;
+}
+
+int check_run_args(hea_problem p, int L, int task, int depth) {
+  if (!p) return fail(HEA_EINVAL, "null problem handle");
+  if (L < 1) return fail(HEA_EINVAL, "solutions must be >= 1");
+  if (task < 1 || task > HEA_MAX_TASK)
+    return fail(HEA_EINVAL, "parallel_task must be in [1, 2048]");
+  if (depth < 0) return fail(HEA_EINVAL, "converge_depth must be >= 0");
+  return HEA_OK;
+}
+
+int mc_lds(hea_problem p, int task, size_t* lds) {
+  const int km = hea_mc_kmax(p->hc.K);
+  *lds = hea_mc_lds_bytes(km, task, p->hc.N, p->sp_lds);
+  if ((int64_t)*lds > p->max_lds) {
+    // fall back to the HBM-resident species path
+    p->sp_lds = false;
+    *lds = hea_mc_lds_bytes(km, task, p->hc.N, false);
+    if ((int64_t)*lds > p->max_lds) return fail(HEA_EUNSUPPORTED, "parallel_task too large for LDS");
+  }
+  HEA_HIP(hea_prepare_local_mc(p->hc.K, p->sp_lds, *lds));
+  return HEA_OK;
+}
+
+void sum_stats(const std::vector<int64_t>& st, int L, hea_stats* out) {
+  hea_stats s = {0, 0, 0, 0, -1.0};
+  for (int l = 0; l < L; ++l) {
+    s.rounds += st[4 * l + 0];
+    s.accepts += st[4 * l + 1];
+    s.evals += st[4 * l + 2];
+    s.draws += st[4 * l + 3];
+  }
+  *out = s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hea_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* hea_last_error(void) { return g_err.c_str(); }
+
+int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) {
+  if (!out) return fail(HEA_EINVAL, "null output handle");
+  *out = nullptr;
+  std::string why;
+  if (!check_problem(d, why)) return fail(HEA_EINVAL, why);
+  const int N = d->n_sites, K = d->n_species, S = d->n_shells;
+  int sumZ = 0;
+  for (int s = 0; s < S; ++s) sumZ += d->shell_size[s];
+  for (size_t e = 0; e < (size_t)N * sumZ; ++e)
+    if (d->nbr[e] < 0 || d->nbr[e] >= N) return fail(HEA_EINVAL, "neighbour index out of range");
+  // The neighbour relation must be symmetric with multiplicity so that the
+  // pair counts, and hence gamma, are symmetric.
+  int self = 0;
+  {
+    int o = 0;
+    for (int s = 0; s < S; ++s) {
+      const int z = d->shell_size[s];
+      for (int i = 0; i < N; ++i) {
+        const int32_t* li = d->nbr + (size_t)i * sumZ + o;
+        for (int k = 0; k < z; ++k) {
+          const int j = li[k];
+          if (j == i) self = 1;
+          const int32_t* lj = d->nbr + (size_t)j * sumZ + o;
+          int mij = 0, mji = 0;
+          for (int q = 0; q < z; ++q) {
+            mij += li[q] == j;
+            mji += lj[q] == i;
+          }
+          if (mij != mji)
+            return fail(HEA_EUNSUPPORTED, "neighbour relation is not symmetric (shell " +
+                                              std::to_string(s) + ")");
+        }
+      }
+      o += z;
+    }
+  }
+  int dev_count = hea_device_count();
+  if (dev_count <= 0) return fail(HEA_ENODEV, "no GPU visible to the HIP runtime");
+  if (device < 0 || device >= dev_count) return fail(HEA_EINVAL, "device index out of range");
+
+  hea_problem p = new hea_problem_s();
+  p->device = device;
+  HeaConst& c = p->hc;
+  memset(&c, 0, sizeof(c));
+  c.N = N;
+  c.K = K;
+  c.S = S;
+  int off = 0;
+  for (int s = 0; s < S; ++s) {
+    c.Z[s] = d->shell_size[s];
+    c.off[s] = off;
+    off += (c.Z[s] + 3) & ~3;
+  }
+  c.stride = off;
+  c.self_nbr = self;
+  c.prefix[0] = 0;
+  for (int a = 0; a < K; ++a) {
+    c.conc[a] = static_cast<double>(d->counts[a]) / static_cast<double>(N);
+    c.prefix[a + 1] = c.prefix[a] + d->counts[a];
+  }
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) c.dd[i][j] = c.conc[i] * c.conc[j];
+  for (int s = 0; s < S; ++s) {
+    c.weight[s] = d->weight[s];
+    c.delta[s] = 1.0 / (double)(c.Z[s] * N);
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j < K; ++j) c.target[s][i][j] = d->target[(size_t)s * K * K + i * K + j];
+    if (!build_segments(c.delta[s], (int64_t)c.Z[s] * N, c.seg_k0[s], c.seg_v0[s], c.seg_inc[s],
+                        c.nseg[s])) {
+      delete p;
+      return fail(HEA_EUNSUPPORTED, "gamma sum has too many binade segments");
+    }
+  }
+  std::vector<int32_t> padded((size_t)N * c.stride, -1);
+  {
+    int o = 0;
+    for (int s = 0; s < S; ++s) {
+      for (int i = 0; i < N; ++i)
+        memcpy(&padded[(size_t)i * c.stride + c.off[s]], d->nbr + (size_t)i * sumZ + o,
+               sizeof(int32_t) * c.Z[s]);
+      o += c.Z[s];
+    }
+  }
+  int rc = HEA_OK;
+  do {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
+    int lds_max = 0;
+    e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
+    p->max_lds = lds_max;
+    if (p->d_const.alloc(1) != hipSuccess || p->d_nbr.alloc(padded.size()) != hipSuccess) {
+      rc = fail(HEA_ENOMEM, "device allocation failed");
+      break;
+    }
+    e = hipMemcpy(p->d_const.p, &c, sizeof(c), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(p->d_nbr.p, padded.data(), sizeof(int32_t) * padded.size(),
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
+  } while (0);
+  if (rc != HEA_OK) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return HEA_OK;
+}
+
+int hea_problem_destroy(hea_problem p) {
+  if (!p) return HEA_OK;
+  (void)hipSetDevice(p->device);
+  delete p;
+  return HEA_OK;
+}
+
+int hea_fitness(hea_problem p, int32_t L, const uint8_t* lattices, double* out) {
+  if (!p || L < 1 || !lattices || !out) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(p->device));
+  int rc = ensure_scratch(p, L);
+  if (rc) return rc;
+  const size_t n = (size_t)L * p->hc.N;
+  HEA_HIP(hipMemcpy(p->d_lat.p, lattices, n, hipMemcpyHostToDevice));
+  if ((rc = launch_recount(p, p->d_lat.p, p->d_counts.p, L, nullptr))) return rc;
+  if ((rc = launch_fitness(p, p->d_counts.p, p->d_fit.p, L, nullptr))) return rc;
+  HEA_HIP(hipMemcpy(out, p->d_fit.p, sizeof(double) * L, hipMemcpyDeviceToHost));
+  return HEA_OK;
+}
+
+int hea_pair_counts(hea_problem p, int32_t L, const uint8_t* lattices, int32_t* out) {
+  if (!p || L < 1 || !lattices || !out) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(p->device));
+  int rc = ensure_scratch(p, L);
+  if (rc) return rc;
+  HEA_HIP(hipMemcpy(p->d_lat.p, lattices, (size_t)L * p->hc.N, hipMemcpyHostToDevice));
+  if ((rc = launch_recount(p, p->d_lat.p, p->d_counts.p, L, nullptr))) return rc;
+  const size_t cn = (size_t)L * p->hc.S * p->hc.K * p->hc.K;
+  HEA_HIP(hipMemcpy(out, p->d_counts.p, sizeof(int32_t) * cn, hipMemcpyDeviceToHost));
+  return HEA_OK;
+}
+
+int hea_generate(hea_problem p, int32_t L, uint64_t seed, uint32_t iter, uint32_t phase,
+                 uint8_t* out_lat, double* out_fit) {
+  if (!p || L < 1 || !out_lat) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(p->device));
+  int rc = ensure_scratch(p, L);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, nullptr, p->d_iter.p, iter);
+  dim3 g((p->hc.N + 255) / 256, L);
+  hipLaunchKernelGGL(k_permute, g, dim3(256), 0, nullptr, p->d_const.p, p->d_lat.p, seed,
+                     p->d_iter.p, phase, 0);
+  HEA_HIP(hipGetLastError());
+  if (out_fit) {
+    if ((rc = launch_recount(p, p->d_lat.p, p->d_counts.p, L, nullptr))) return rc;
+    if ((rc = launch_fitness(p, p->d_counts.p, p->d_fit.p, L, nullptr))) return rc;
+    HEA_HIP(hipMemcpy(out_fit, p->d_fit.p, sizeof(double) * L, hipMemcpyDeviceToHost));
+  }
+  HEA_HIP(hipMemcpy(out_lat, p->d_lat.p, (size_t)L * p->hc.N, hipMemcpyDeviceToHost));
+  return HEA_OK;
+}
+
+static int local_mc_dev(hea_problem p, int L, uint8_t* lat, double* fit, int task, int depth,
+                        uint64_t seed, uint32_t iter, uint32_t phase, int sort, hipStream_t st) {
+  int rc = ensure_scratch(p, L);
+  if (rc) return rc;
+  size_t lds;
+  if ((rc = mc_lds(p, task, &lds))) return rc;
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p->d_iter.p, iter);
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st,
+                     reinterpret_cast<int32_t*>(p->d_stats.p), (size_t)8 * L);
+  if ((rc = launch_recount(p, lat, p->d_counts.p, L, st))) return rc;
+  if ((rc = launch_mc(p, lat, fit, p->d_counts.p, p->d_stats.p, p->d_iter.p, nullptr, L, task,
+                      depth, seed, phase, -1.0, lds, st)))
+    return rc;
+  if (sort) {
+    if ((rc = launch_merge(lat, fit, L, lat, fit, 0, L, p->d_perm.p, p->d_lat2.p, p->d_fit2.p,
+                           p->hc.N, st)))
+      return rc;
+    HEA_HIP(hipMemcpyAsync(lat, p->d_lat2.p, (size_t)L * p->hc.N, hipMemcpyDeviceToDevice, st));
+    HEA_HIP(hipMemcpyAsync(fit, p->d_fit2.p, sizeof(double) * L, hipMemcpyDeviceToDevice, st));
+  }
+  return HEA_OK;
+}
+
+int hea_local_mc(hea_problem p, int32_t L, uint8_t* lattices, double* fitness, int32_t task,
+                 int32_t depth, uint64_t seed, uint32_t iter, uint32_t phase, int32_t sort,
+                 hea_stats* stats) {
+  int rc = check_run_args(p, L, task, depth);
+  if (rc) return rc;
+  if (!lattices || !fitness) return fail(HEA_EINVAL, "null population");
+  HEA_HIP(hipSetDevice(p->device));
+  const size_t n = (size_t)L * p->hc.N;
+  DevBuf<uint8_t> lat;
+  DevBuf<double> fit;
+  HEA_HIP(lat.alloc(n));
+  HEA_HIP(fit.alloc(L));
+  HEA_HIP(hipMemcpy(lat.p, lattices, n, hipMemcpyHostToDevice));
+  HEA_HIP(hipMemcpy(fit.p, fitness, sizeof(double) * L, hipMemcpyHostToDevice));
+  if ((rc = local_mc_dev(p, L, lat.p, fit.p, task, depth, seed, iter, phase, sort, nullptr)))
+    return rc;
+  HEA_HIP(hipDeviceSynchronize());
+  HEA_HIP(hipMemcpy(lattices, lat.p, n, hipMemcpyDeviceToHost));
+  HEA_HIP(hipMemcpy(fitness, fit.p, sizeof(double) * L, hipMemcpyDeviceToHost));
+  if (stats) {
+    std::vector<int64_t> st((size_t)4 * L);
+    HEA_HIP(hipMemcpy(st.data(), p->d_stats.p, sizeof(int64_t) * st.size(),
+                      hipMemcpyDeviceToHost));
+    sum_stats(st, L, stats);
+  }
+  return HEA_OK;
+}
+
+int hea_local_mc_device(hea_problem p, int32_t L, void* d_lattices, void* d_fitness, int32_t task,
+                        int32_t depth, uint64_t seed, uint32_t iter, uint32_t phase,
+                        void* stream) {
+  int rc = check_run_args(p, L, task, depth);
+  if (rc) return rc;
+  if (!d_lattices || !d_fitness) return fail(HEA_EINVAL, "null population");
+  HEA_HIP(hipSetDevice(p->device));
+  return local_mc_dev(p, L, static_cast<uint8_t*>(d_lattices), static_cast<double*>(d_fitness),
+                      task, depth, seed, iter, phase, 0, static_cast<hipStream_t>(stream));
+}
+
+// ------------------------------------------------------------------ session --
+
+static int session_enqueue_iteration(hea_session s) {
+  hea_problem p = s->p;
+  const int L = s->L, N = p->hc.N;
+  hipStream_t st = s->stream;
+  int rc;
+  // new population (accelerate.cpp:743)
+  dim3 g((N + 255) / 256, L);
+  hipLaunchKernelGGL(k_permute, g, dim3(256), 0, st, p->d_const.p, s->B.p, s->seed, s->iter.p,
+                     (uint32_t)HEA_PHASE_NEW, 0);
+  if ((rc = launch_recount(p, s->B.p, s->counts.p, L, st))) return rc;
+  if ((rc = launch_fitness(p, s->counts.p, s->fB.p, L, st))) return rc;
+  // local search I + its closing sort (accelerate.cpp:745, 693)
+  if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
+                      s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, st)))
+    return rc;
+  if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->C.p, s->fC.p,
+                         N, st)))
+    return rc;
+  // merge old ++ new, keep the best L (accelerate.cpp:747, 305-330)
+  if ((rc = launch_merge(s->A.p, s->fA.p, L, s->C.p, s->fC.p, L, L, s->perm.p, s->B.p, s->fB.p, N,
+                         st)))
+    return rc;
+  // local search II + sort (accelerate.cpp:749, 693)
+  if ((rc = launch_recount(p, s->B.p, s->counts.p, L, st))) return rc;
+  if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
+                      s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, st)))
+    return rc;
+  if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->A.p, s->fA.p,
+                         N, st)))
+    return rc;
+  hipLaunchKernelGGL(k_iter_add, dim3(1), dim3(1), 0, st, s->iter.p, 1u);
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, double thr,
+                       uint64_t seed, hea_session* out) {
+  if (!out) return fail(HEA_EINVAL, "null output handle");
+  *out = nullptr;
+  int rc = check_run_args(p, L, task, depth);
+  if (rc) return rc;
+  HEA_HIP(hipSetDevice(p->device));
+  hea_session s = new hea_session_s();
+  s->p = p;
+  s->L = L;
+  s->task = task;
+  s->depth = depth;
+  s->thr = thr;
+  s->seed = seed;
+  const size_t n = (size_t)L * p->hc.N;
+  const size_t cn = (size_t)L * p->hc.S * p->hc.K * p->hc.K;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = s->A.alloc(n);
+  if (e == hipSuccess) e = s->B.alloc(n);
+  if (e == hipSuccess) e = s->C.alloc(n);
+  if (e == hipSuccess) e = s->fA.alloc(L);
+  if (e == hipSuccess) e = s->fB.alloc(L);
+  if (e == hipSuccess) e = s->fC.alloc(L);
+  if (e == hipSuccess) e = s->counts.alloc(cn);
+  if (e == hipSuccess) e = s->perm.alloc((size_t)2 * L);
+  if (e == hipSuccess) e = s->stats.alloc((size_t)4 * L);
+  if (e == hipSuccess) e = s->iter.alloc(1);
+  if (e == hipSuccess) e = s->clk.alloc(2);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(HEA_ENOMEM, std::string("session allocation: ") + hipGetErrorString(e));
+  }
+  if ((rc = mc_lds(p, task, &s->lds))) {
+    delete s;
+    return rc;
+  }
+  e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(HEA_EHIP, hipGetErrorString(e));
+  }
+  *out = s;
+  return HEA_OK;
+}
+
+int hea_session_init(hea_session s) {
+  if (!s) return fail(HEA_EINVAL, "null session");
+  hea_problem p = s->p;
+  HEA_HIP(hipSetDevice(p->device));
+  const int L = s->L;
+  hipStream_t st = s->stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st,
+                     reinterpret_cast<int32_t*>(s->stats.p), (size_t)8 * L);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, s->iter.p, 0u);
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, s->clk.p + 1, ~0ull);
+  hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, st, s->clk.p);
+  // initial population (accelerate.cpp:737)
+  dim3 g((p->hc.N + 255) / 256, L);
+  hipLaunchKernelGGL(k_permute, g, dim3(256), 0, st, p->d_const.p, s->A.p, s->seed, s->iter.p,
+                     (uint32_t)HEA_PHASE_INIT, 0);
+  int rc;
+  if ((rc = launch_recount(p, s->A.p, s->counts.p, L, st))) return rc;
+  if ((rc = launch_fitness(p, s->counts.p, s->fA.p, L, st))) return rc;
+  HEA_HIP(hipGetLastError());
+  s->iters_done = 0;
+  return HEA_OK;
+}
+
+int hea_session_step(hea_session s, int32_t n_iter) {
+  if (!s || n_iter < 0) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(s->p->device));
+  if (!s->exec) {
+    HEA_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = session_enqueue_iteration(s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(HEA_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    s->graph = g;
+    HEA_HIP(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < n_iter; ++i) HEA_HIP(hipGraphLaunch(s->exec, s->stream));
+  s->iters_done += n_iter;
+  return HEA_OK;
+}
+
+int hea_session_sync(hea_session s) {
+  if (!s) return fail(HEA_EINVAL, "null session");
+  HEA_HIP(hipSetDevice(s->p->device));
+  HEA_HIP(hipStreamSynchronize(s->stream));
+  return HEA_OK;
+}
+
+int hea_session_best(hea_session s, double* best) {
+  if (!s || !best) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(s->p->device));
+  HEA_HIP(hipStreamSynchronize(s->stream));
+  HEA_HIP(hipMemcpy(best, s->fA.p, sizeof(double), hipMemcpyDeviceToHost));
+  return HEA_OK;
+}
+
+int hea_session_read(hea_session s, uint8_t* out_lat, double* out_fit, hea_stats* stats) {
+  if (!s) return fail(HEA_EINVAL, "null session");
+  hea_problem p = s->p;
+  HEA_HIP(hipSetDevice(p->device));
+  HEA_HIP(hipStreamSynchronize(s->stream));
+  const int L = s->L;
+  if (out_lat) HEA_HIP(hipMemcpy(out_lat, s->A.p, (size_t)L * p->hc.N, hipMemcpyDeviceToHost));
+  if (out_fit) HEA_HIP(hipMemcpy(out_fit, s->fA.p, sizeof(double) * L, hipMemcpyDeviceToHost));
+  if (stats) {
+    std::vector<int64_t> st((size_t)4 * L);
+    HEA_HIP(hipMemcpy(st.data(), s->stats.p, sizeof(int64_t) * st.size(),
+                      hipMemcpyDeviceToHost));
+    sum_stats(st, L, stats);
+    unsigned long long clk[2];
+    HEA_HIP(hipMemcpy(clk, s->clk.p, sizeof(clk), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HEA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, p->device));
+    if (clk[1] != ~0ull && khz > 0)
+      stats->first_hit_s = (double)(clk[1] - clk[0]) / (1000.0 * khz);
+  }
+  return HEA_OK;
+}
+
+int hea_session_destroy(hea_session s) {
+  if (!s) return HEA_OK;
+  (void)hipSetDevice(s->p->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return HEA_OK;
+}
+
+int hea_hcs_run(hea_problem p, int32_t L, int32_t total_iter, int32_t task, int32_t depth,
+                double thr, uint64_t seed, uint8_t* out_lat, double* out_fit, hea_stats* stats) {
+  if (total_iter < 0) return fail(HEA_EINVAL, "total_iter must be >= 0");
+  hea_session s = nullptr;
+  int rc = hea_session_create(p, L, task, depth, thr, seed, &s);
+  if (rc) return rc;
+  if (!(rc = hea_session_init(s)) && !(rc = hea_session_step(s, total_iter)))
+    rc = hea_session_read(s, out_lat, out_fit, stats);
+  hea_session_destroy(s);
+  return rc;
+}
+
+}  // extern "C"

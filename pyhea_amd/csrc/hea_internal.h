@@ -1,0 +1,67 @@
+// Internal layout shared by the HIP kernels and the host side of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hea_stream.h"
+
+#define HEA_MAX_K 8        // species per lattice handled by the kernels
+#define HEA_MAX_S 3        // neighbour shells (the reference builds 3)
+#define HEA_MAX_Z 31       // members per shell (5-bit packed counters)
+#define HEA_MC_THREADS 256  // threads of one local-search workgroup
+#define HEA_MAX_TASK 2048  // proposals per round (parallel_task)
+#define HEA_MAX_SEG 160    // binade segments of one shell's R_s(k) table
+
+// Per-problem constants, resident in HBM and read through the scalar cache.
+//
+// gamma entries of a fresh recount equal R_s(k): the k-fold sequential FP64
+// sum of delta_s = 1/(Z_s*N) (pyhea/cpp/accelerate.cpp:160, 395).  R_s is
+// piecewise linear: within a run of equal increments R_s(k) = v0 + (k-k0)*inc
+// exactly, so it is stored as segments (k0, v0, inc) found by walking the
+// sum once on the host.
+struct HeaConst {
+  int32_t N, K, S, stride;       // stride: ints per neighbour row (multiple of 4)
+  int32_t Z[HEA_MAX_S];          // shell sizes
+  int32_t off[HEA_MAX_S];        // first column of each shell in a row (multiple of 4)
+  int32_t nseg[HEA_MAX_S];
+  int32_t self_nbr;              // 1 if some site lists itself (tiny cells)
+  double delta[HEA_MAX_S];       // fl(1.0/(Z_s*N))
+  double weight[HEA_MAX_S];
+  double conc[HEA_MAX_K];           // fl(count_i/N)
+  int32_t prefix[HEA_MAX_K + 1];     // prefix sums of the species counts
+  double dd[HEA_MAX_K][HEA_MAX_K];  // fl(fl(c_i/N)*fl(c_j/N)), accelerate.cpp:562
+  double target[HEA_MAX_S][HEA_MAX_K][HEA_MAX_K];
+  int64_t seg_k0[HEA_MAX_S][HEA_MAX_SEG];
+  double seg_v0[HEA_MAX_S][HEA_MAX_SEG];
+  double seg_inc[HEA_MAX_S][HEA_MAX_SEG];
+};
+
+// Arguments of one local-search launch over a population of L lattices.
+struct HeaMcArgs {
+  const HeaConst* pc;
+  const int32_t* nbr;       // [N][stride], per-shell lists ascending, padding -1
+  uint8_t* lattices;        // [L][N] in/out
+  double* fitness;          // [L] in/out (stored fitness = prev_fit)
+  const int32_t* counts;    // [L][S][K][K] integer pair counts of the input lattices
+  int64_t* stats;           // [L][4] += rounds, accepts, evals, draws
+  const uint32_t* iter_ptr; // stream iteration (device memory, so graphs can replay)
+  unsigned long long* first_hit;  // wall clock of the first fitness <= thr (atomicMin)
+  uint64_t seed;
+  int32_t L, task, depth;
+  uint32_t phase;
+  double thr;
+};
+
+// R_s(k) from the binade segments (exact: the host verified every k).
+__device__ __forceinline__ double hea_r_of_count(const HeaConst& P, int s, int64_t k) {
+  int q = 0;
+  while (q + 1 < P.nseg[s] && P.seg_k0[s][q + 1] <= k) ++q;
+  return fma((double)(k - P.seg_k0[s][q]), P.seg_inc[s][q], P.seg_v0[s][q]);
+}
+
+// Launchers implemented in hea_mc.hip.
+size_t hea_mc_lds_bytes(int kmax, int task, int n_sites, bool sp_lds);
+int hea_mc_kmax(int K);
+hipError_t hea_prepare_local_mc(int K, bool sp_lds, size_t lds);
+hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, size_t lds,
+                               hipStream_t st);
