@@ -1,0 +1,126 @@
+"""Writes the golden vectors in tests/golden/ from the reference engine.
+
+Runs oracle/_ref/ref_hcs -- pyhea/cpp/accelerate.cpp compiled in place by
+``make -C oracle ref`` with the keyed streams injected (oracle/ref_harness.cpp)
+-- on tiny problems and stores inputs and outputs (FP64 as uint64 bit
+patterns) in golden_*.npz.  Only data is kept; no reference source.
+
+    python tests/golden/make_golden.py
+
+Cases cover: FCC (both shell sets) and BCC, K = 2..6, S = 1..3, 1x1x1 and
+2x2x2 cells (self neighbours and repeated neighbours), task in {0, 1, 4, 8},
+depth 1..4, swap chains with adjacent sites, and the full global loop.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from oracle import Oracle, RefHarness  # noqa: E402
+from pyhea_amd.problem import make_problem  # noqa: E402
+
+PROBLEMS = [
+    # name, type, cell, fractions, shells, S
+    ("fcc3_k3", "FCC", 3, [0.4, 0.3, 0.3], "pyhea", 3),
+    ("fcc3_k4_true", "FCC", 3, [0.25, 0.25, 0.25, 0.25], "true", 3),
+    ("bcc4_k3", "BCC", 4, [0.5, 0.25, 0.25], "true", 3),
+    ("bcc3_k5_s2", "BCC", 3, [0.2] * 5, "pyhea", 2),
+    ("fcc2_k2", "FCC", 2, [0.5, 0.5], "true", 3),
+    ("fcc1_k2", "FCC", 1, [0.5, 0.5], "pyhea", 3),
+    ("bcc2_k2_s1", "BCC", 2, [0.5, 0.5], "pyhea", 1),
+    ("fcc4_k6", "FCC", 4, [0.3, 0.2, 0.2, 0.1, 0.1, 0.1], "true", 3),
+    ("bcc4_k4_s1", "BCC", 4, [0.25] * 4, "pyhea", 1),
+]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def problem_arrays(name, typ, cell, fr, shells, S, seed):
+    spec = make_problem(typ, cell, fr, weight=[4.0, 1.0, 0.5][:S], shells=shells,
+                        target_seed=seed)
+    return spec, {
+        f"{name}/nbr": spec.nbr.astype(np.int32),
+        f"{name}/shell_sizes": np.array(spec.shell_sizes, np.int32),
+        f"{name}/counts": np.array(spec.counts, np.int32),
+        f"{name}/weight": np.array(spec.weight, np.float64),
+        f"{name}/target": np.asarray(spec.target, np.float64),
+    }
+
+
+def adjacent_swaps(o: Oracle, lat, n, rng):
+    """Swap chain with about one third of the swaps between neighbours."""
+    lat = lat.copy()
+    sw = []
+    while len(sw) < n:
+        a1 = int(rng.integers(0, o.N))
+        if rng.random() < 0.35:
+            a2 = int(o.nbr[a1, rng.integers(0, o.nbr.shape[1])])
+        else:
+            a2 = int(rng.integers(0, o.N))
+        if lat[a1] != lat[a2]:
+            sw.append((a1, a2))
+            lat[a1], lat[a2] = lat[a2], lat[a1]
+    return np.array(sw, np.int32)
+
+
+def main():
+    if not RefHarness.available():
+        raise SystemExit("oracle/_ref/ref_hcs missing: run `make -C oracle ref` first")
+    arrays, index = {}, {"problems": [], "local_mc": [], "run_hcs": []}
+    for pi, (name, typ, cell, fr, shells, S) in enumerate(PROBLEMS):
+        spec, arrs = problem_arrays(name, typ, cell, fr, shells, S, seed=100 + pi)
+        arrays.update(arrs)
+        o = Oracle.from_spec(spec)
+        r = RefHarness(o)
+        index["problems"].append(name)
+        lats = np.stack([o.permute(1000 + pi, 0, 0, slot) for slot in range(6)])
+        arrays[f"{name}/fit_lats"] = lats
+        arrays[f"{name}/fit_bits"] = bits(r.fitness(lats))
+        rng = np.random.default_rng(pi)
+        sw = adjacent_swaps(o, lats[0], 200, rng)
+        fits, gam, lat_out = r.incremental_chain(lats[0], sw)
+        arrays[f"{name}/chain_lat0"] = lats[0]
+        arrays[f"{name}/chain_swaps"] = sw
+        arrays[f"{name}/chain_fit_bits"] = bits(fits)
+        arrays[f"{name}/chain_gamma_bits"] = bits(gam)
+        arrays[f"{name}/chain_lat_out"] = lat_out
+        for task, depth in [(1, 1), (4, 3), (8, 2), (8, 4), (0, 2)]:
+            key = f"{name}/mc_t{task}_d{depth}"
+            lat_in = lats[:4]
+            fit_in = r.fitness(lat_in)
+            seed, it, ph = 7 + pi, 1, 2 + (task % 2)
+            lo, fo, words, _ = r.local_mc(lat_in, fit_in, task, depth, seed, it, ph)
+            arrays[key + "/lat_in"] = lat_in
+            arrays[key + "/fit_in_bits"] = bits(fit_in)
+            arrays[key + "/lat_out"] = lo
+            arrays[key + "/fit_out_bits"] = bits(fo)
+            arrays[key + "/words"] = words
+            index["local_mc"].append(dict(problem=name, key=key, task=task, depth=depth,
+                                          seed=seed, iter=it, phase=ph))
+        if pi < 6:
+            key = f"{name}/hcs"
+            L, I, T, D, seed = 5, 2, 8, 2, 50 + pi
+            snaps, words, _ = r.run_hcs(L, I, T, D, seed)
+            arrays[key + "/lat"] = np.stack([s[0] for s in snaps])
+            arrays[key + "/fit_bits"] = np.stack([bits(s[1]) for s in snaps])
+            arrays[key + "/words"] = words
+            index["run_hcs"].append(dict(problem=name, key=key, solutions=L, total_iter=I,
+                                         task=T, depth=D, seed=seed))
+    np.savez_compressed(os.path.join(HERE, "golden_reference.npz"), **arrays)
+    with open(os.path.join(HERE, "golden_index.json"), "w") as f:
+        json.dump(index, f, indent=1)
+    print("wrote", len(arrays), "arrays")
+
+
+if __name__ == "__main__":
+    main()
