@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark of the SRO lattice search on one MI355X.
+
+Workload (default ``plain-bcc16``): the plain drop-in search on a BCC 16^3
+lattice (8192 sites), 5 species at fractions [0.30, 0.25, 0.20, 0.15, 0.10],
+3 shells (8/6/12), weights [4, 1, 0.5], seeded targets, solutions = 512,
+parallel_task = 256, converge_depth = 10.  One step = one global iteration
+of run_local_parallel_hcs (pyhea/cpp/accelerate.cpp:741-761): generate,
+local search I, merge, local search II, replayed as one HIP graph with the
+population resident in HBM.  --warmup 1 --steps 49 covers total_iter = 50.
+
+Prints one JSON line: swap evals/s (value), the roofline of the dominant
+kernel (the local-search kernel, timed live with HIP events on its stream)
+and a CPU baseline: the reference engine (oracle/_ref/ref_hcs) timed on a
+bounded sample of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def bytes_per_eval(sum_z: int) -> int:
+    """Algorithmic bytes of one swap eval: both sites' neighbour indices
+    (int32) and neighbour species (uint8), plus the two sites' species."""
+    return 2 * sum_z * (4 + 1) + 2
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=49)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="plain-bcc16", choices=["plain-bcc16", "plain-fcc10"])
+    ap.add_argument("--profile-steps", type=int, default=3,
+                    help="eager iterations timed per launch for the roofline")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="approximate CPU work of the baseline sample (0 disables it)")
+    ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (a copy "
+                         "kernel of known size for calibrating HBM byte counters)")
+    return ap.parse_args()
+
+
+def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int):
+    """Reference engine (or the oracle when the reference binary is absent)
+    running local search I on lattices of the same workload, one process per
+    core.  Eval counts come from the GPU run of the identical work (the two
+    are bit-identical, so the counts are the reference's)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, RefHarness
+
+    from pyhea_amd.hcs import Problem
+
+    o = Oracle.from_spec(spec)
+    kind = "reference" if RefHarness.available() else "port"
+    # per-lattice cost of local search I measured on the reference engine
+    # in this repository's container: ~0.6 s for BCC 16^3, T = 256
+    per_lat = 0.6
+    n_lat = max(procs, int(round(target_secs / per_lat)))
+    n_lat = (n_lat + procs - 1) // procs * procs
+    lat0 = np.stack([o.permute(run["seed"], 0, 1, l) for l in range(n_lat)])
+    fit0 = o.fitness(lat0)
+    T, D = run["parallel_task"], run["converge_depth"]
+    chunks = np.array_split(np.arange(n_lat), procs)
+    with tempfile.TemporaryDirectory() as tmp:
+        jobs = []
+        t0 = time.perf_counter()
+        for ci, idx in enumerate(chunks):
+            f = os.path.join(tmp, f"c{ci}.npz")
+            np.savez(f, lat=lat0[idx], fit=fit0[idx])
+            jobs.append(subprocess.Popen([sys.executable, os.path.abspath(__file__),
+                                          "--cpu-worker", f, kind, str(T), str(D),
+                                          str(run["seed"]), str(wid)]))
+        for j in jobs:
+            if j.wait() != 0:
+                raise RuntimeError("cpu baseline process failed")
+        wall = time.perf_counter() - t0
+    p = Problem.from_spec(spec)
+    _, _, st = p.local_mc(lat0, fit0, T, D, run["seed"], 0, 2, sort=False)
+    return {
+        "value": st["evals"] / wall,
+        "unit": "swap evals/s",
+        "cores": procs,
+        "kind": kind,
+        "sample": f"local search I (T={T}, depth={D}) on {n_lat} random "
+                  f"{spec.latt_typ} {spec.cell_dim}^3 lattices, "
+                  f"{procs} processes, {st['evals']} evals in {wall:.1f} s",
+    }
+
+
+def cpu_worker(path: str, kind: str, T: int, D: int, seed: int, wid: int) -> None:
+    """One CPU-baseline process: local search I on its share of lattices."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, RefHarness
+
+    from pyhea_amd.problem import workload
+
+    spec, _ = workload(wid)
+    o = Oracle.from_spec(spec)
+    d = np.load(path)
+    if kind == "reference":
+        RefHarness(o).local_mc(d["lat"], d["fit"], T, D, seed, 0, 2)
+    else:
+        o.local_mc(d["lat"], d["fit"], T, D, seed, 0, 2, sort=False)
+
+
+def main():
+    if len(sys.argv) == 8 and sys.argv[1] == "--cpu-worker":
+        cpu_worker(sys.argv[2], sys.argv[3], *map(int, sys.argv[4:8]))
+        return
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the population would shard one sub-population per GPU)"}))
+        sys.exit(2)
+    from pyhea_amd import _native
+    from pyhea_amd.hcs import Problem
+    from pyhea_amd.problem import workload
+
+    if _native.device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    wid = {"plain-bcc16": 1, "plain-fcc10": 0}[a.workload]
+    spec, run = workload(wid)
+    L, T, D, seed = run["solutions"], run["parallel_task"], run["converge_depth"], run["seed"]
+    prob = Problem.from_spec(spec)
+    sess = prob.session(L, T, D, thr=1e-3, seed=seed)
+    t_start = time.perf_counter()
+    sess.init()
+    sess.step(a.warmup)
+    sess.sync()
+    ev0 = sess.read()[2]["evals"]
+    t0 = time.perf_counter()
+    sess.step(a.steps)
+    sess.sync()
+    t1 = time.perf_counter()
+    lat, fit, st = sess.read()
+    evals = st["evals"] - ev0
+    secs = t1 - t0
+    value = evals / secs
+
+    # roofline of the dominant kernel: per-launch times with HIP events
+    ms, ev = sess.step_timed(a.profile_steps)
+    bpe = bytes_per_eval(spec.sum_z)
+    achieved = float((ev.sum() * bpe) / (ms.sum() * 1e-3) / 1e9)
+    n_launch = ms.size
+    roof = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "k_local_mc",
+        "bytes_per_eval": bpe,
+        "avg_launch_ms": round(float(ms.mean()), 4),
+        "evals_per_launch": int(ev.sum() // n_launch),
+    }
+    if a.calib_copy_mb > 0:
+        import torch
+
+        n = a.calib_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+    cpu = None
+    if a.cpu_seconds > 0:
+        cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, a.cpu_procs)
+    out = {
+        "metric": "swap evals/s",
+        "value": value,
+        "unit": "swap evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": secs / max(a.steps, 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": a.workload,
+            "lattice": f"{spec.latt_typ} {spec.cell_dim}^3",
+            "n_sites": spec.n_sites,
+            "species": spec.counts,
+            "shells": spec.shell_sizes,
+            "solutions": L,
+            "parallel_task": T,
+            "converge_depth": D,
+            "total_iter": a.warmup + a.steps,
+            "seed": seed,
+        },
+        "best_fitness": float(fit[0]),
+        "time_to_fitness_1e-3_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,
+        "wall_s": time.perf_counter() - t_start,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

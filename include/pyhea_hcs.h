@@ -133,6 +133,12 @@ int hea_session_create(hea_problem p, int32_t solutions, int32_t parallel_task,
 int hea_session_init(hea_session s);
 /* Runs `n_iter` global iterations asynchronously on the session stream. */
 int hea_session_step(hea_session s, int32_t n_iter);
+/* Measurement helper with no counterpart in the reference: runs n_iter
+ * global iterations eagerly (no graph) with HIP events around each local
+ * search.  out_ms[2i], out_ms[2i+1]: durations (ms) of local search I and
+ * II of iteration i; out_evals[2i], out_evals[2i+1]: their swap evals.
+ * Synchronous. */
+int hea_session_step_timed(hea_session s, int32_t n_iter, float* out_ms, int64_t* out_evals);
 /* Waits for the session stream. */
 int hea_session_sync(hea_session s);
 /* Copies out the population (sorted) and the accumulated counters. */

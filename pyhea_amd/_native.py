@@ -14,7 +14,8 @@ import threading
 HEA_OK, HEA_EINVAL, HEA_ENODEV, HEA_EHIP, HEA_EUNSUPPORTED, HEA_ENOMEM = range(6)
 PHASE_INIT, PHASE_NEW, PHASE_LS1, PHASE_LS2 = range(4)
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpyhea_hcs.so")
+LIB_PATH = os.environ.get(
+    "PYHEA_AMD_LIB", os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpyhea_hcs.so"))
 
 # Every symbol declared by include/pyhea_hcs.h.
 EXPORTS = (
@@ -31,6 +32,7 @@ EXPORTS = (
     "hea_session_create",
     "hea_session_init",
     "hea_session_step",
+    "hea_session_step_timed",
     "hea_session_sync",
     "hea_session_read",
     "hea_session_best",
@@ -94,6 +96,8 @@ def _declare(lib: C.CDLL) -> None:
         "hea_session_create": (C.c_int, [P, I32, I32, I32, D, U64, C.POINTER(P)]),
         "hea_session_init": (C.c_int, [P]),
         "hea_session_step": (C.c_int, [P, I32]),
+        "hea_session_step_timed": (C.c_int, [P, I32, C.POINTER(C.c_float),
+                                             C.POINTER(C.c_int64)]),
         "hea_session_sync": (C.c_int, [P]),
         "hea_session_read": (C.c_int, [P, u8p, dp, C.POINTER(HeaStats)]),
         "hea_session_best": (C.c_int, [P, dp]),

@@ -9,6 +9,7 @@
 //   calculate_best_lattices       305-330 -> k_rank (old ++ new) + k_gather
 //   run_local_parallel_hcs        713-763 -> hea_session_* (one graph/iteration)
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -183,7 +184,7 @@ struct DevBuf {
 struct hea_problem_s {
   int device = 0;
   HeaConst hc;
-  bool sp_lds = true;
+  bool force_hbm_species = false;  // PYHEA_AMD_SPECIES_IN_HBM=1 (tests)
   int64_t max_lds = 0;
   DevBuf<HeaConst> d_const;
   DevBuf<int32_t> d_nbr;
@@ -204,6 +205,7 @@ struct hea_session_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   size_t lds = 0;
+  bool sp_lds = true;
   DevBuf<uint8_t> A, B, C;
   DevBuf<double> fA, fB, fC;
   DevBuf<int32_t> counts, perm;
@@ -322,7 +324,8 @@ int launch_merge(const uint8_t* A, const double* fa, int na, const uint8_t* B, c
 
 int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, int64_t* stats,
               const uint32_t* iter_ptr, unsigned long long* first_hit, int L, int task,
-              int depth, uint64_t seed, uint32_t phase, double thr, size_t lds, hipStream_t st) {
+              int depth, uint64_t seed, uint32_t phase, double thr, size_t lds, bool sp_lds,
+              hipStream_t st) {
   HeaMcArgs a;
   a.pc = p->d_const.p;
   a.nbr = p->d_nbr.p;
@@ -336,9 +339,21 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.L = L;
   a.task = task;
   a.depth = depth;
+  a.tp = hea_mc_tp(task, p->hc.S);
   a.phase = phase;
   a.thr = thr;
-  HEA_HIP(hea_launch_local_mc(a, p->hc.K, p->sp_lds, lds, st));
+  const HeaConst& c = p->hc;
+  a.N = c.N;
+  a.K = c.K;
+  a.S = c.S;
+  a.stride = c.stride;
+  for (int s = 0; s < HEA_MAX_S; ++s) {
+    a.Z[s] = c.Z[s];
+    a.off[s] = c.off[s];
+    a.delta[s] = c.delta[s];
+    a.weight[s] = c.weight[s];
+  }
+  HEA_HIP(hea_launch_local_mc(a, p->hc.K, sp_lds, lds, st));
   return HEA_OK;
 }
 
@@ -351,16 +366,18 @@ int check_run_args(hea_problem p, int L, int task, int depth) {
   return HEA_OK;
 }
 
-int mc_lds(hea_problem p, int task, size_t* lds) {
+// LDS bytes of a local-search launch; species live in LDS when they fit.
+int mc_lds(hea_problem p, int task, size_t* lds, bool* sp_lds) {
   const int km = hea_mc_kmax(p->hc.K);
-  *lds = hea_mc_lds_bytes(km, task, p->hc.N, p->sp_lds);
+  const int tp = hea_mc_tp(task, p->hc.S);
+  *sp_lds = !p->force_hbm_species;
+  *lds = hea_mc_lds_bytes(km, task, tp, p->hc.N, *sp_lds);
   if ((int64_t)*lds > p->max_lds) {
-    // fall back to the HBM-resident species path
-    p->sp_lds = false;
-    *lds = hea_mc_lds_bytes(km, task, p->hc.N, false);
+    *sp_lds = false;  // species read from HBM
+    *lds = hea_mc_lds_bytes(km, task, tp, p->hc.N, false);
     if ((int64_t)*lds > p->max_lds) return fail(HEA_EUNSUPPORTED, "parallel_task too large for LDS");
   }
-  HEA_HIP(hea_prepare_local_mc(p->hc.K, p->sp_lds, *lds));
+  HEA_HIP(hea_prepare_local_mc(p->hc.K, *sp_lds, *lds));
   return HEA_OK;
 }
 
@@ -429,6 +446,10 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
 
   hea_problem p = new hea_problem_s();
   p->device = device;
+  {
+    const char* v = getenv("PYHEA_AMD_SPECIES_IN_HBM");
+    p->force_hbm_species = v && v[0] == '1';
+  }
   HeaConst& c = p->hc;
   memset(&c, 0, sizeof(c));
   c.N = N;
@@ -553,13 +574,14 @@ static int local_mc_dev(hea_problem p, int L, uint8_t* lat, double* fit, int tas
   int rc = ensure_scratch(p, L);
   if (rc) return rc;
   size_t lds;
-  if ((rc = mc_lds(p, task, &lds))) return rc;
+  bool sp_lds;
+  if ((rc = mc_lds(p, task, &lds, &sp_lds))) return rc;
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p->d_iter.p, iter);
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st,
                      reinterpret_cast<int32_t*>(p->d_stats.p), (size_t)8 * L);
   if ((rc = launch_recount(p, lat, p->d_counts.p, L, st))) return rc;
   if ((rc = launch_mc(p, lat, fit, p->d_counts.p, p->d_stats.p, p->d_iter.p, nullptr, L, task,
-                      depth, seed, phase, -1.0, lds, st)))
+                      depth, seed, phase, -1.0, lds, sp_lds, st)))
     return rc;
   if (sort) {
     if ((rc = launch_merge(lat, fit, L, lat, fit, 0, L, p->d_perm.p, p->d_lat2.p, p->d_fit2.p,
@@ -612,7 +634,14 @@ int hea_local_mc_device(hea_problem p, int32_t L, void* d_lattices, void* d_fitn
 
 // ------------------------------------------------------------------ session --
 
-static int session_enqueue_iteration(hea_session s) {
+// Optional measurement hooks of one iteration: events around both local
+// searches and stats snapshots before/after them (eager runs only).
+struct IterHooks {
+  hipEvent_t ev[4];
+  int64_t* snap;  // [3][4L]
+};
+
+static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullptr) {
   hea_problem p = s->p;
   const int L = s->L, N = p->hc.N;
   hipStream_t st = s->stream;
@@ -624,9 +653,18 @@ static int session_enqueue_iteration(hea_session s) {
   if ((rc = launch_recount(p, s->B.p, s->counts.p, L, st))) return rc;
   if ((rc = launch_fitness(p, s->counts.p, s->fB.p, L, st))) return rc;
   // local search I + its closing sort (accelerate.cpp:745, 693)
+  const size_t sb = sizeof(int64_t) * 4 * L;
+  if (hk) {
+    HEA_HIP(hipMemcpyAsync(hk->snap, s->stats.p, sb, hipMemcpyDeviceToDevice, st));
+    HEA_HIP(hipEventRecord(hk->ev[0], st));
+  }
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
-                      s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, st)))
+                      s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, s->sp_lds, st)))
     return rc;
+  if (hk) {
+    HEA_HIP(hipEventRecord(hk->ev[1], st));
+    HEA_HIP(hipMemcpyAsync(hk->snap + 4 * L, s->stats.p, sb, hipMemcpyDeviceToDevice, st));
+  }
   if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->C.p, s->fC.p,
                          N, st)))
     return rc;
@@ -636,9 +674,14 @@ static int session_enqueue_iteration(hea_session s) {
     return rc;
   // local search II + sort (accelerate.cpp:749, 693)
   if ((rc = launch_recount(p, s->B.p, s->counts.p, L, st))) return rc;
+  if (hk) HEA_HIP(hipEventRecord(hk->ev[2], st));
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
-                      s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, st)))
+                      s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, s->sp_lds, st)))
     return rc;
+  if (hk) {
+    HEA_HIP(hipEventRecord(hk->ev[3], st));
+    HEA_HIP(hipMemcpyAsync(hk->snap + 8 * L, s->stats.p, sb, hipMemcpyDeviceToDevice, st));
+  }
   if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->A.p, s->fA.p,
                          N, st)))
     return rc;
@@ -679,7 +722,7 @@ int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, do
     delete s;
     return fail(HEA_ENOMEM, std::string("session allocation: ") + hipGetErrorString(e));
   }
-  if ((rc = mc_lds(p, task, &s->lds))) {
+  if ((rc = mc_lds(p, task, &s->lds, &s->sp_lds))) {
     delete s;
     return rc;
   }
@@ -734,6 +777,39 @@ int hea_session_step(hea_session s, int32_t n_iter) {
   for (int i = 0; i < n_iter; ++i) HEA_HIP(hipGraphLaunch(s->exec, s->stream));
   s->iters_done += n_iter;
   return HEA_OK;
+}
+
+int hea_session_step_timed(hea_session s, int32_t n_iter, float* out_ms, int64_t* out_evals) {
+  if (!s || n_iter < 0 || !out_ms || !out_evals) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(s->p->device));
+  const int L = s->L;
+  IterHooks hk;
+  for (auto& e : hk.ev) HEA_HIP(hipEventCreate(&e));
+  DevBuf<int64_t> snap;
+  HEA_HIP(snap.alloc((size_t)12 * L));
+  hk.snap = snap.p;
+  std::vector<int64_t> h((size_t)12 * L);
+  int rc = HEA_OK;
+  for (int i = 0; i < n_iter && rc == HEA_OK; ++i) {
+    if ((rc = session_enqueue_iteration(s, &hk))) break;
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&out_ms[2 * i], hk.ev[0], hk.ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&out_ms[2 * i + 1], hk.ev[2], hk.ev[3]);
+    if (e == hipSuccess)
+      e = hipMemcpy(h.data(), snap.p, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(HEA_EHIP, hipGetErrorString(e));
+      break;
+    }
+    int64_t ev[3] = {0, 0, 0};
+    for (int q = 0; q < 3; ++q)
+      for (int l = 0; l < L; ++l) ev[q] += h[(size_t)q * 4 * L + 4 * l + 2];
+    out_evals[2 * i] = ev[1] - ev[0];
+    out_evals[2 * i + 1] = ev[2] - ev[1];
+    ++s->iters_done;
+  }
+  for (auto& e : hk.ev) (void)hipEventDestroy(e);
+  return rc;
 }
 
 int hea_session_sync(hea_session s) {

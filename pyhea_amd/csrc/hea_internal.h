@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "hea_stream.h"
 
 #define HEA_MAX_K 8        // species per lattice handled by the kernels
@@ -47,9 +49,13 @@ struct HeaMcArgs {
   const uint32_t* iter_ptr; // stream iteration (device memory, so graphs can replay)
   unsigned long long* first_hit;  // wall clock of the first fitness <= thr (atomicMin)
   uint64_t seed;
-  int32_t L, task, depth;
+  int32_t L, task, depth, tp;   // tp: proposals per pass (lanes per shell)
   uint32_t phase;
   double thr;
+  // problem scalars, copied from HeaConst so they live in the kernarg segment
+  int32_t N, K, S, stride;
+  int32_t Z[HEA_MAX_S], off[HEA_MAX_S];
+  double delta[HEA_MAX_S], weight[HEA_MAX_S];
 };
 
 // R_s(k) from the binade segments (exact: the host verified every k).
@@ -60,8 +66,9 @@ __device__ __forceinline__ double hea_r_of_count(const HeaConst& P, int s, int64
 }
 
 // Launchers implemented in hea_mc.hip.
-size_t hea_mc_lds_bytes(int kmax, int task, int n_sites, bool sp_lds);
+size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds);
 int hea_mc_kmax(int K);
+int hea_mc_tp(int task, int S);
 hipError_t hea_prepare_local_mc(int K, bool sp_lds, size_t lds);
 hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, size_t lds,
                                hipStream_t st);

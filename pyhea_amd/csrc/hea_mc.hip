@@ -12,6 +12,15 @@
 //     drifted gamma forward (accelerate.cpp:682);
 //   * the search ends after `depth` consecutive rejected rounds.
 //
+// Work layout.  A workgroup has S x TP lanes: lane (s, j) evaluates shell s
+// of proposal j (TP = proposals per pass, a multiple of 64, so every wave
+// holds one shell and walks neighbour lists of one uniform length).  Each
+// lane produces its shell's partial sum e_s = sum_ij (alpha_sij - t_sij)^2 in
+// the reference's (i, j) order; the shell-0 lane then forms
+// err = ((0 + w0 e0) + w1 e1) + w2 e2 and F = sqrt(err) exactly as
+// accelerate.cpp:557-569 does.  The lattice's species sit in LDS when they
+// fit, otherwise they are read from HBM.
+//
 // Exact FP64 bookkeeping.  The reference updates gamma by +-delta_s with
 // delta_s = 1/(Z_s*N) (accelerate.cpp:526-548); the result depends on the
 // order of those additions.  Every update to entry (p,q) is paired with the
@@ -29,11 +38,8 @@
 // g + net*r exactly, computed by one fma.  Entries whose window of reachable
 // values is not provably inside one binade, or whose step ties, are marked
 // r = 0 and replayed addition by addition.  Either way the result equals the
-// reference's doubles bit for bit.
-//
-// The fitness sum runs over all K*K*S terms in the reference's (s, i, j)
-// order (accelerate.cpp:557-569); untouched terms come from a per-round
-// table.  No FP contraction is allowed in this file (-ffp-contract=off).
+// reference's doubles bit for bit.  No FP contraction is allowed in this file
+// (-ffp-contract=off).
 
 #include <type_traits>
 
@@ -41,12 +47,12 @@
 
 namespace {
 
-constexpr int kWaves = HEA_MC_THREADS / 64;
+constexpr int kMaxWaves = 16;  // 1024 lanes
 
 struct McShared {
-  int wcount[kWaves];
-  unsigned long long wkey[kWaves];
-  int wj[kWaves];
+  int wcount[kMaxWaves];
+  unsigned long long wkey[kMaxWaves];
+  int wj[kMaxWaves];
   uint32_t next_pos;
   int pad;
 };
@@ -86,28 +92,31 @@ __device__ __forceinline__ double replay(double v, double d1, int n1, int n2) {
   return v;
 }
 
-template <int KMAX>
-struct Tables {
-  double* G;   // [3][KMAX][KMAX] prev_gamma
-  double* E0;  // [3][KMAX][KMAX] (1 - G/d - t)^2 of prev_gamma
-  double* R;   // [3][KMAX][KMAX] safe step or 0
-  double* conc;  // [KMAX] fl(count/N)
-  __device__ int idx(int s, int i, int j) const { return (s * KMAX + i) * KMAX + j; }
-};
-
-template <int KMAX>
-__device__ void refresh_entry(const HeaConst& P, const Tables<KMAX>& T, int s, int i, int j) {
-  const int e = T.idx(s, i, j);
-  const double g = T.G[e];
-  const double d = T.conc[i] * T.conc[j];
-  const double sro = 1.0 - g / d;
-  const double df = sro - P.target[s][i][j];
-  T.E0[e] = df * df;
-  T.R[e] = safe_step(g, P.delta[s], P.Z[s]);
+// Exact replay of entry (a,b) for proposal (a1, a2) over one shell
+// (accelerate.cpp:517-549 restricted to the updates that hit (a,b)).
+__device__ __forceinline__ double replay_ab(double v, double delta, const int32_t* __restrict__ r1,
+                                            const int32_t* __restrict__ r2, int z,
+                                            const uint8_t* __restrict__ sp, int a1, int a2, int a,
+                                            int b) {
+  for (int k = 0; k < z; ++k) {
+    const int nb = r1[k];
+    const int n = sp[nb];
+    const int na = (nb == a2) ? a : (nb == a1) ? b : n;
+    if (n == b) v = v - delta;   // gamma[a][n]  -= d
+    if (na == a) v = v + delta;  // gamma[na][b] += d
+  }
+  for (int k = 0; k < z; ++k) {
+    const int nb = r2[k];
+    const int m = sp[nb];
+    const int ma = (nb == a1) ? b : (nb == a2) ? a : m;
+    if (m == a) v = v - delta;   // gamma[m][b]  -= d
+    if (ma == b) v = v + delta;  // gamma[a][ma] += d
+  }
+  return v;
 }
 
-// Counts of one neighbour walk: M += 1<<(5*species), cs counts `self`,
-// co counts `other`.
+// Species histogram of one neighbour list: M += 1<<(W*species); cs counts
+// `self`, co counts `other`.  Rows are padded to a multiple of 4 entries.
 template <int KMAX>
 __device__ __forceinline__ void walk(const int32_t* __restrict__ row, int z,
                                      const uint8_t* __restrict__ sp, int self, int other,
@@ -130,116 +139,144 @@ __device__ __forceinline__ void walk(const int32_t* __restrict__ row, int z,
   }
 }
 
-// Exact replay of entry (a,b) of shell s for proposal (a1, a2)
-// (accelerate.cpp:517-549 restricted to the updates that hit (a,b)).
-__device__ __noinline__ double replay_ab(double v, double delta, const int32_t* __restrict__ r1,
-                                         const int32_t* __restrict__ r2, int z,
-                                         const uint8_t* __restrict__ sp, int a1, int a2, int a,
-                                         int b) {
-  for (int k = 0; k < z; ++k) {
-    const int nb = r1[k];
-    const int n = sp[nb];
-    const int na = (nb == a2) ? a : (nb == a1) ? b : n;
-    if (n == b) v = v - delta;   // gamma[a][n]  -= d
-    if (na == a) v = v + delta;  // gamma[na][b] += d
+// LDS carve-up of one workgroup.
+template <int KMAX>
+struct Lds {
+  static constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
+  double* G;     // [S][KMAX][KMAX] prev_gamma
+  double* E0;    // [S][KMAX][KMAX] (1 - G/d - t)^2 of prev_gamma
+  double* R;     // [S][KMAX][KMAX] safe step or 0
+  double* tgt;   // [S][KMAX][KMAX] target alpha
+  double* conc;  // [KMAX] fl(count/N)
+  double* esum;  // [S][TP] per-shell partial sums
+  int2* prop;    // [task]
+  McShared* sh;
+  uint8_t* sp;   // [N] when resident
+  static __host__ __device__ size_t bytes(int task, int tp, int n, bool sp_lds) {
+    size_t b = sizeof(double) * (4 * TAB + KMAX + HEA_MAX_S * (size_t)tp);
+    b += sizeof(int2) * (size_t)task + sizeof(McShared);
+    if (sp_lds) b += ((size_t)n + 15) & ~(size_t)15;
+    return b;
   }
-  for (int k = 0; k < z; ++k) {
-    const int nb = r2[k];
-    const int m = sp[nb];
-    const int ma = (nb == a1) ? b : (nb == a2) ? a : m;
-    if (m == a) v = v - delta;   // gamma[m][b]  -= d
-    if (ma == b) v = v + delta;  // gamma[a][ma] += d
+  __device__ void carve(unsigned char* base, int task, int tp, bool sp_lds) {
+    G = reinterpret_cast<double*>(base);
+    E0 = G + TAB;
+    R = E0 + TAB;
+    tgt = R + TAB;
+    conc = tgt + TAB;
+    esum = conc + KMAX;
+    prop = reinterpret_cast<int2*>(esum + HEA_MAX_S * tp);
+    sh = reinterpret_cast<McShared*>(prop + task);
+    sp = sp_lds ? reinterpret_cast<uint8_t*>(sh + 1) : nullptr;
   }
-  return v;
+  __device__ __forceinline__ int idx(int s, int i, int j) const { return (s * KMAX + i) * KMAX + j; }
+};
+
+template <int KMAX>
+__device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int i, int j) {
+  const int e = T.idx(s, i, j);
+  const double g = T.G[e];
+  const double d = T.conc[i] * T.conc[j];
+  const double sro = 1.0 - g / d;
+  const double df = sro - T.tgt[e];
+  T.E0[e] = df * df;
+  T.R[e] = safe_step(g, A.delta[s], A.Z[s]);
 }
 
-// Incremental fitness of swapping a1 <-> a2 (species a != b) against
-// prev_gamma; fills the swapped gamma rows a (nA) and b (nB) per shell.
-template <int KMAX, bool SP_LDS>
-__device__ __forceinline__ double eval_swap(const HeaConst& P, const Tables<KMAX>& T,
-                                            const int32_t* __restrict__ nbr,
-                                            const uint8_t* __restrict__ sp, int a1, int a2,
-                                            double (&nA)[HEA_MAX_S][KMAX],
-                                            double (&nB)[HEA_MAX_S][KMAX]) {
-  const int K = P.K;
+// Shell s of the incremental fitness of swapping a1 <-> a2 (species a != b)
+// against prev_gamma.  Returns the shell's partial sum and fills the swapped
+// gamma rows a (nA) and b (nB) of that shell.
+template <int KMAX>
+__device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>& T,
+                                             const uint8_t* __restrict__ sp, int s, int a1,
+                                             int a2, double (&nA)[KMAX], double (&nB)[KMAX]) {
+  using PK = Pack<KMAX>;
+  const int K = A.K;
+  const int z = A.Z[s];
+  const double delta = A.delta[s];
   const int a = sp[a1], b = sp[a2];
-  const double conc_a = T.conc[a], conc_b = T.conc[b];
-  double err = 0.0;
-#pragma unroll
-  for (int s = 0; s < HEA_MAX_S; ++s) {
-    if (s < P.S) {
-      const int z = P.Z[s];
-      const double delta = P.delta[s];
-      const int32_t* r1 = nbr + (size_t)a1 * P.stride + P.off[s];
-      const int32_t* r2 = nbr + (size_t)a2 * P.stride + P.off[s];
-      using PK = Pack<KMAX>;
-      typename PK::T MA, MB;
-      int c11, c12, c22, c21;
-      walk<KMAX>(r1, z, sp, a1, a2, MA, c11, c12);
-      walk<KMAX>(r2, z, sp, a2, a1, MB, c22, c21);
-      const int maA = PK::fld(MA, a), maB = PK::fld(MA, b);
-      const int mbA = PK::fld(MB, a), mbB = PK::fld(MB, b);
+  const int32_t* r1 = A.nbr + (size_t)a1 * A.stride + A.off[s];
+  const int32_t* r2 = A.nbr + (size_t)a2 * A.stride + A.off[s];
+  typename PK::T MA, MB;
+  int c11, c12, c22, c21;
+  walk<KMAX>(r1, z, sp, a1, a2, MA, c11, c12);
+  walk<KMAX>(r2, z, sp, a2, a1, MB, c22, c21);
+  const int maA = PK::fld(MA, a), maB = PK::fld(MA, b);
+  const int mbA = PK::fld(MB, a), mbB = PK::fld(MB, b);
 
-      // the three entries with interleaved or doubled updates
-      const int eAA = T.idx(s, a, a), eBB = T.idx(s, b, b), eAB = T.idx(s, a, b);
-      const int pAA = 2 * (mbA - c21 + c22), mAA = 2 * maA;
-      const int pBB = 2 * (maB - c12 + c11), mBB = 2 * mbB;
-      const int netAB = (maA + c12 - c11) + (mbB + c21 - c22) - maB - mbA;
-      double gAA = T.G[eAA], rAA = T.R[eAA];
-      double gBB = T.G[eBB], rBB = T.R[eBB];
-      double gAB = T.G[eAB], rAB = T.R[eAB];
-      double vAA = fma((double)(pAA - mAA), rAA, gAA);
-      if (rAA == 0.0) vAA = replay(gAA, -delta, mAA, pAA);
-      double vBB = fma((double)(pBB - mBB), rBB, gBB);
-      if (rBB == 0.0) vBB = replay(gBB, delta, pBB, mBB);
-      double vAB = fma((double)netAB, rAB, gAB);
-      if (rAB == 0.0) vAB = replay_ab(gAB, delta, r1, r2, z, sp, a1, a2, a, b);
-
-      double sA[KMAX], sB[KMAX];
+  // fast path for every touched entry; collect the ones needing replay
+  const int eAA = T.idx(s, a, a), eBB = T.idx(s, b, b), eAB = T.idx(s, a, b);
+  const int pAA = 2 * (mbA - c21 + c22), mAA = 2 * maA;
+  const int pBB = 2 * (maB - c12 + c11), mBB = 2 * mbB;
+  const int netAB = (maA + c12 - c11) + (mbB + c21 - c22) - maB - mbA;
+  const double gAA = T.G[eAA], rAA = T.R[eAA];
+  const double gBB = T.G[eBB], rBB = T.R[eBB];
+  const double gAB = T.G[eAB], rAB = T.R[eAB];
+  double vAA = fma((double)(pAA - mAA), rAA, gAA);
+  double vBB = fma((double)(pBB - mBB), rBB, gBB);
+  double vAB = fma((double)netAB, rAB, gAB);
+  unsigned need = (rAA == 0.0 ? 1u : 0u) | (rBB == 0.0 ? 2u : 0u) | (rAB == 0.0 ? 4u : 0u);
+  int ma[KMAX], mb[KMAX];
 #pragma unroll
-      for (int x = 0; x < KMAX; ++x) {
-        if (x < K) {
-          const int ma = PK::fld(MA, x), mb = PK::fld(MB, x);
-          const int eA = T.idx(s, a, x), eB = T.idx(s, b, x);
-          const double gA = T.G[eA], rA = T.R[eA];
-          const double gB = T.G[eB], rB = T.R[eB];
-          double vA = fma((double)(mb - ma), rA, gA);
-          double vB = fma((double)(ma - mb), rB, gB);
-          const bool gen = (x != a) && (x != b);
-          if (gen && rA == 0.0) vA = replay(gA, -delta, ma, mb);
-          if (gen && rB == 0.0) vB = replay(gB, delta, ma, mb);
-          vA = (x == a) ? vAA : (x == b) ? vAB : vA;
-          vB = (x == b) ? vBB : (x == a) ? vAB : vB;
-          nA[s][x] = vA;
-          nB[s][x] = vB;
-          sA[x] = 1.0 - vA / (conc_a * T.conc[x]);
-          sB[x] = 1.0 - vB / (conc_b * T.conc[x]);
-        }
-      }
-
-      // sum over all K*K terms in row-major order (accelerate.cpp:560-566)
-      double es = 0.0;
-#pragma unroll
-      for (int i = 0; i < KMAX; ++i) {
-        if (i < K) {
-          const bool ia = (i == a), ib = (i == b);
-#pragma unroll
-          for (int j = 0; j < KMAX; ++j) {
-            if (j < K) {
-              const bool ja = (j == a), jb = (j == b);
-              const double sro = ia ? sA[j] : ib ? sB[j] : ja ? sA[i] : sB[i];
-              const double df = sro - P.target[s][i][j];
-              const double et = df * df;
-              const double e = (ia || ib || ja || jb) ? et : T.E0[T.idx(s, i, j)];
-              es = es + e;
-            }
-          }
-        }
-      }
-      err = err + P.weight[s] * es;
+  for (int x = 0; x < KMAX; ++x) {
+    ma[x] = PK::fld(MA, x);
+    mb[x] = PK::fld(MB, x);
+    if (x < K) {
+      const int eA = T.idx(s, a, x), eB = T.idx(s, b, x);
+      const double gA = T.G[eA], rA = T.R[eA];
+      const double gB = T.G[eB], rB = T.R[eB];
+      nA[x] = fma((double)(mb[x] - ma[x]), rA, gA);
+      nB[x] = fma((double)(ma[x] - mb[x]), rB, gB);
+      const bool gen = (x != a) && (x != b);
+      need |= (gen && rA == 0.0) ? (8u << x) : 0u;
+      need |= (gen && rB == 0.0) ? (2048u << x) : 0u;
     }
   }
-  return sqrt(err);
+  if (need) {  // rare: entries near a binade edge, tied steps or zero counts
+    if (need & 1u) vAA = replay(gAA, -delta, mAA, pAA);
+    if (need & 2u) vBB = replay(gBB, delta, pBB, mBB);
+    if (need & 4u) vAB = replay_ab(gAB, delta, r1, r2, z, sp, a1, a2, a, b);
+#pragma unroll
+    for (int x = 0; x < KMAX; ++x) {
+      if (need & (8u << x)) nA[x] = replay(T.G[T.idx(s, a, x)], -delta, ma[x], mb[x]);
+      if (need & (2048u << x)) nB[x] = replay(T.G[T.idx(s, b, x)], delta, ma[x], mb[x]);
+    }
+  }
+
+  const double conc_a = T.conc[a], conc_b = T.conc[b];
+  double sA[KMAX], sB[KMAX];
+#pragma unroll
+  for (int x = 0; x < KMAX; ++x) {
+    if (x < K) {
+      nA[x] = (x == a) ? vAA : (x == b) ? vAB : nA[x];
+      nB[x] = (x == b) ? vBB : (x == a) ? vAB : nB[x];
+      sA[x] = 1.0 - nA[x] / (conc_a * T.conc[x]);
+      sB[x] = 1.0 - nB[x] / (conc_b * T.conc[x]);
+    }
+  }
+
+  // all K*K terms in row-major order (accelerate.cpp:560-566)
+  const double* tg = T.tgt + T.idx(s, 0, 0);
+  const double* e0 = T.E0 + T.idx(s, 0, 0);
+  double es = 0.0;
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) {
+    if (i < K) {
+      const bool ia = (i == a), ib = (i == b);
+#pragma unroll
+      for (int j = 0; j < KMAX; ++j) {
+        if (j < K) {
+          const bool ja = (j == a), jb = (j == b);
+          const double sro = ia ? sA[j] : ib ? sB[j] : ja ? sA[i] : sB[i];
+          const double df = sro - tg[i * KMAX + j];
+          const double et = df * df;
+          const double e = (ia || ib || ja || jb) ? et : e0[i * KMAX + j];
+          es = es + e;
+        }
+      }
+    }
+  }
+  return es;
 }
 
 __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned long long f2,
@@ -248,42 +285,42 @@ __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned
 }
 
 template <int KMAX, bool SP_LDS>
-__global__ __launch_bounds__(HEA_MC_THREADS) void k_local_mc(HeaMcArgs A) {
+__global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const HeaConst& P = *A.pc;
   const int l = blockIdx.x;
+  const int nthr = blockDim.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int N = P.N, K = P.K, S = P.S;
-  const int task = A.task;
+  const int N = A.N, K = A.K, S = A.S;
+  const int task = A.task, TP = A.tp;
+  const int nwaves = nthr >> 6;
+  const int s_me = __builtin_amdgcn_readfirstlane(tid / TP);  // this wave's shell
+  const int jl = tid - s_me * TP;                             // proposal slot in a pass
+  const int nq = (task + TP - 1) / TP;                        // passes per round
 
-  constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
-  Tables<KMAX> T;
-  T.G = reinterpret_cast<double*>(smem);
-  T.E0 = T.G + TAB;
-  T.R = T.E0 + TAB;
-  T.conc = T.R + TAB;
-  int2* prop = reinterpret_cast<int2*>(T.conc + KMAX);
-  McShared* sh = reinterpret_cast<McShared*>(prop + task);
+  Lds<KMAX> T;
+  T.carve(smem, task, TP, SP_LDS);
   uint8_t* lat_g = A.lattices + (size_t)l * N;
   uint8_t* sp;
   if constexpr (SP_LDS) {
-    sp = reinterpret_cast<uint8_t*>(sh + 1);
-    for (int i = tid; i < N; i += HEA_MC_THREADS) sp[i] = lat_g[i];
+    sp = T.sp;
+    for (int i = tid; i < N; i += nthr) sp[i] = lat_g[i];
   } else {
     sp = lat_g;
   }
 
-  // prev_gamma = R_s(pair counts) (accelerate.cpp:636-640)
+  // prev_gamma = R_s(pair counts) (accelerate.cpp:636-640) and constants
   const int32_t* cnt = A.counts + (size_t)l * S * K * K;
-  for (int e = tid; e < S * K * K; e += HEA_MC_THREADS) {
+  for (int e = tid; e < S * K * K; e += nthr) {
     const int s = e / (K * K), i = (e / K) % K, j = e % K;
     T.G[T.idx(s, i, j)] = hea_r_of_count(P, s, cnt[e]);
+    T.tgt[T.idx(s, i, j)] = P.target[s][i][j];
   }
   if (tid < K) T.conc[tid] = P.conc[tid];
   __syncthreads();
-  for (int e = tid; e < S * K * K; e += HEA_MC_THREADS) {
+  for (int e = tid; e < S * K * K; e += nthr) {
     const int s = e / (K * K), i = (e / K) % K, j = e % K;
-    refresh_entry(P, T, s, i, j);
+    refresh_entry(A, T, s, i, j);
   }
   __syncthreads();
 
@@ -293,10 +330,7 @@ __global__ __launch_bounds__(HEA_MC_THREADS) void k_local_mc(HeaMcArgs A) {
   int acc = 0;
   long long rounds = 0, accepts = 0;
   bool hit_thr = false;
-
-  // gamma rows a and b after the lane's proposal; with more proposals than
-  // lanes only the last one is kept and the owner re-evaluates on accept.
-  double nA[HEA_MAX_S][KMAX], nB[HEA_MAX_S][KMAX];
+  double nA[KMAX], nB[KMAX];  // this lane's shell rows after its last proposal
 
   while (acc < A.depth) {
     // ---- draw `task` valid proposals in stream order (accelerate.cpp:649-655)
@@ -308,99 +342,105 @@ __global__ __launch_bounds__(HEA_MC_THREADS) void k_local_mc(HeaMcArgs A) {
       const bool valid = sp[s1] != sp[s2];
       const unsigned long long bal = __ballot(valid);
       const int pre = __popcll(bal & ((1ull << lane) - 1ull));
-      if (lane == 0) sh->wcount[wave] = __popcll(bal);
+      if (lane == 0) T.sh->wcount[wave] = __popcll(bal);
       __syncthreads();
       int off = 0, tot = 0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        const int c = sh->wcount[w];
+      for (int w = 0; w < nwaves; ++w) {
+        const int c = T.sh->wcount[w];
         off += (w < wave) ? c : 0;
         tot += c;
       }
       const int rank = found + off + pre;
-      if (valid && rank < task) prop[rank] = make_int2((int)s1, (int)s2);
-      if (valid && rank == task - 1) sh->next_pos = p + 1;
+      if (valid && rank < task) T.prop[rank] = make_int2((int)s1, (int)s2);
+      if (valid && rank == task - 1) T.sh->next_pos = p + 1;
       found += tot;
-      pos += HEA_MC_THREADS;
+      pos += nthr;
       __syncthreads();
     }
-    pos = sh->next_pos;
+    pos = T.sh->next_pos;
 
     // ---- evaluate; keep the first strict minimum (accelerate.cpp:663-674)
     unsigned long long bkey = ~0ull;
     int bj = 0x7fffffff;
-    for (int j = tid; j < task; j += HEA_MC_THREADS) {
-      const int2 pr = prop[j];
-      const double f = eval_swap<KMAX, SP_LDS>(P, T, A.nbr, sp, pr.x, pr.y, nA, nB);
-      const unsigned long long fk = __double_as_longlong(f);
-      if (fk < bkey) {
-        bkey = fk;
-        bj = j;
+    for (int q = 0; q < nq; ++q) {
+      const int j = jl + q * TP;
+      if (j < task) {
+        const int2 pr = T.prop[j];
+        T.esum[s_me * TP + jl] = eval_shell<KMAX>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
       }
+      __syncthreads();
+      if (s_me == 0 && j < task) {
+        double err = 0.0;
+        for (int s = 0; s < S; ++s) err = err + A.weight[s] * T.esum[s * TP + jl];
+        const unsigned long long fk = __double_as_longlong(sqrt(err));
+        if (fk < bkey) {
+          bkey = fk;
+          bj = j;
+        }
+      }
+      if (q + 1 < nq) __syncthreads();
     }
-    // workgroup argmin of (fitness bits, proposal index); fitness >= 0 so
-    // its bit pattern orders like the value
-    unsigned long long rk = bkey;
-    int rj = bj;
+    // argmin of (fitness bits, proposal index) over the shell-0 lanes;
+    // fitness >= 0, so its bit pattern orders like the value
+    const int nw0 = TP >> 6;
+    if (s_me == 0) {
+      unsigned long long rk = bkey;
+      int rj = bj;
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-      const unsigned long long ok =
-          ((unsigned long long)(uint32_t)__shfl_xor((int)(rk >> 32), m) << 32) |
-          (uint32_t)__shfl_xor((int)(uint32_t)rk, m);
-      const int oj = __shfl_xor(rj, m);
-      if (key_less(ok, oj, rk, rj)) {
-        rk = ok;
-        rj = oj;
+      for (int m = 32; m >= 1; m >>= 1) {
+        const unsigned long long ok =
+            ((unsigned long long)(uint32_t)__shfl_xor((int)(rk >> 32), m) << 32) |
+            (uint32_t)__shfl_xor((int)(uint32_t)rk, m);
+        const int oj = __shfl_xor(rj, m);
+        if (key_less(ok, oj, rk, rj)) {
+          rk = ok;
+          rj = oj;
+        }
       }
-    }
-    if (lane == 0) {
-      sh->wkey[wave] = rk;
-      sh->wj[wave] = rj;
+      if (lane == 0) {
+        T.sh->wkey[wave] = rk;
+        T.sh->wj[wave] = rj;
+      }
     }
     __syncthreads();
-    unsigned long long wk = sh->wkey[0];
-    int wj = sh->wj[0];
-#pragma unroll
-    for (int w = 1; w < kWaves; ++w)
-      if (key_less(sh->wkey[w], sh->wj[w], wk, wj)) {
-        wk = sh->wkey[w];
-        wj = sh->wj[w];
+    unsigned long long wk = T.sh->wkey[0];
+    int wj = T.sh->wj[0];
+    for (int w = 1; w < nw0; ++w)
+      if (key_less(T.sh->wkey[w], T.sh->wj[w], wk, wj)) {
+        wk = T.sh->wkey[w];
+        wj = T.sh->wj[w];
       }
     const double best_fit = __longlong_as_double((long long)wk);
     ++rounds;
 
     // ---- accept iff strictly better (accelerate.cpp:678-689)
     if (best_fit < prev_fit) {
-      if (bj == wj) {  // the owner of the winning proposal writes it back
-        const int2 pr = prop[wj];
-        if (task > HEA_MC_THREADS)
-          (void)eval_swap<KMAX, SP_LDS>(P, T, A.nbr, sp, pr.x, pr.y, nA, nB);
-        const int a = sp[pr.x], b = sp[pr.y];
+      const int2 pr = T.prop[wj];
+      const int a = sp[pr.x], b = sp[pr.y];
+      if (jl == wj % TP) {  // the winner's lane of every shell writes its rows
+        if (nq > 1) (void)eval_shell<KMAX>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
 #pragma unroll
-        for (int s = 0; s < HEA_MAX_S; ++s) {
-          if (s < S) {
-#pragma unroll
-            for (int x = 0; x < KMAX; ++x) {
-              if (x < K) {
-                T.G[T.idx(s, a, x)] = nA[s][x];
-                T.G[T.idx(s, x, a)] = nA[s][x];
-                T.G[T.idx(s, b, x)] = nB[s][x];
-                T.G[T.idx(s, x, b)] = nB[s][x];
-              }
-            }
+        for (int x = 0; x < KMAX; ++x) {
+          if (x < K) {
+            T.G[T.idx(s_me, a, x)] = nA[x];
+            T.G[T.idx(s_me, x, a)] = nA[x];
+            T.G[T.idx(s_me, b, x)] = nB[x];
+            T.G[T.idx(s_me, x, b)] = nB[x];
           }
         }
-        sp[pr.x] = (uint8_t)b;
-        sp[pr.y] = (uint8_t)a;
       }
       prev_fit = best_fit;
       acc = 0;
       ++accepts;
       if (best_fit <= A.thr) hit_thr = true;
       __syncthreads();
-      for (int e = tid; e < S * K * K; e += HEA_MC_THREADS) {
+      if (tid == 0) {
+        sp[pr.x] = (uint8_t)b;
+        sp[pr.y] = (uint8_t)a;
+      }
+      for (int e = tid; e < S * K * K; e += nthr) {
         const int s = e / (K * K), i = (e / K) % K, j = e % K;
-        refresh_entry(P, T, s, i, j);
+        refresh_entry(A, T, s, i, j);
       }
       __syncthreads();
       if (hit_thr && tid == 0 && A.first_hit) {
@@ -413,7 +453,7 @@ __global__ __launch_bounds__(HEA_MC_THREADS) void k_local_mc(HeaMcArgs A) {
   }
 
   if constexpr (SP_LDS) {
-    for (int i = tid; i < N; i += HEA_MC_THREADS) lat_g[i] = sp[i];
+    for (int i = tid; i < N; i += nthr) lat_g[i] = sp[i];
   }
   if (tid == 0) {
     A.fitness[l] = prev_fit;
@@ -425,26 +465,12 @@ __global__ __launch_bounds__(HEA_MC_THREADS) void k_local_mc(HeaMcArgs A) {
   }
 }
 
-}  // namespace
-
-// Shared-memory bytes of one local-search workgroup.
-size_t hea_mc_lds_bytes(int kmax, int task, int n_sites, bool sp_lds) {
-  size_t b = sizeof(double) * (3 * HEA_MAX_S * kmax * kmax + kmax);
-  b += sizeof(int2) * task + sizeof(McShared);
-  if (sp_lds) b += ((size_t)n_sites + 15) & ~(size_t)15;
-  return b;
-}
-
-int hea_mc_kmax(int K) {
-  return K <= 2 ? 2 : K <= 3 ? 3 : K <= 4 ? 4 : K <= 5 ? 5 : K <= 6 ? 6 : K <= 8 ? 8 : 0;
-}
-
 template <int KMAX, bool SP_LDS>
-static const void* mc_fn() {
+const void* mc_fn() {
   return reinterpret_cast<const void*>(k_local_mc<KMAX, SP_LDS>);
 }
 
-static const void* mc_fn_for(int K, bool sp_lds) {
+const void* mc_fn_for(int K, bool sp_lds) {
   switch (hea_mc_kmax(K)) {
     case 2: return sp_lds ? mc_fn<2, true>() : mc_fn<2, false>();
     case 3: return sp_lds ? mc_fn<3, true>() : mc_fn<3, false>();
@@ -453,6 +479,31 @@ static const void* mc_fn_for(int K, bool sp_lds) {
     case 6: return sp_lds ? mc_fn<6, true>() : mc_fn<6, false>();
     case 8: return sp_lds ? mc_fn<8, true>() : mc_fn<8, false>();
     default: return nullptr;
+  }
+}
+
+}  // namespace
+
+int hea_mc_kmax(int K) {
+  return K <= 2 ? 2 : K <= 3 ? 3 : K <= 4 ? 4 : K <= 5 ? 5 : K <= 6 ? 6 : K <= 8 ? 8 : 0;
+}
+
+// Proposals evaluated per pass (lanes per shell): a multiple of 64, at most
+// 256, with S * TP <= 1024 lanes.
+int hea_mc_tp(int task, int S) {
+  const int want = ((task + 63) / 64) * 64;
+  const int cap = std::min(256, (1024 / S) / 64 * 64);
+  return std::min(want, cap);
+}
+
+size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds) {
+  switch (kmax) {
+    case 2: return Lds<2>::bytes(task, tp, n_sites, sp_lds);
+    case 3: return Lds<3>::bytes(task, tp, n_sites, sp_lds);
+    case 4: return Lds<4>::bytes(task, tp, n_sites, sp_lds);
+    case 5: return Lds<5>::bytes(task, tp, n_sites, sp_lds);
+    case 6: return Lds<6>::bytes(task, tp, n_sites, sp_lds);
+    default: return Lds<8>::bytes(task, tp, n_sites, sp_lds);
   }
 }
 
@@ -469,5 +520,5 @@ hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, size_t ld
   const void* fn = mc_fn_for(K, sp_lds);
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<HeaMcArgs*>(&a)};
-  return hipLaunchKernel(fn, dim3(a.L), dim3(HEA_MC_THREADS), args, lds, st);
+  return hipLaunchKernel(fn, dim3(a.L), dim3(a.S * a.tp), args, lds, st);
 }

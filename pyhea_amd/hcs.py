@@ -139,6 +139,15 @@ class Session:
     def step(self, n_iter: int = 1) -> None:
         N.check(N.lib().hea_session_step(self._h, n_iter))
 
+    def step_timed(self, n_iter: int = 1):
+        """Eager iterations with per-launch kernel times: (ms [n,2], evals [n,2])."""
+        ms = np.zeros((n_iter, 2), dtype=np.float32)
+        ev = np.zeros((n_iter, 2), dtype=np.int64)
+        N.check(N.lib().hea_session_step_timed(self._h, n_iter,
+                                               ms.ctypes.data_as(C.POINTER(C.c_float)),
+                                               ev.ctypes.data_as(C.POINTER(C.c_int64))))
+        return ms, ev
+
     def sync(self) -> None:
         N.check(N.lib().hea_session_sync(self._h))
 
