@@ -59,6 +59,7 @@ typedef struct hea_problem_desc {
 
 typedef struct hea_problem_s* hea_problem;
 typedef struct hea_session_s* hea_session;
+typedef struct hea_batch_s* hea_batch;
 
 /* Counters of the work done; one swap eval = one incremental delta-F of a
  * valid proposal. */
@@ -131,6 +132,14 @@ int hea_session_create(hea_problem p, int32_t solutions, int32_t parallel_task,
                        int32_t converge_depth, double thr, uint64_t seed, hea_session* out);
 /* Initial population (accelerate.cpp:737); starts the session clock. */
 int hea_session_init(hea_session s);
+/* Guided proposals, no counterpart in the reference: every `every` rounds a
+ * local search recomputes site gains u[k][c] (first-order change of the
+ * fitness squared when site k turns into species c, from the relaxation's
+ * gradient at the current lattice); each of the parallel_task proposals of a
+ * round is the Gumbel-max of -(u[i][b] + u[j][a]) / (tau * scale) over
+ * `candidates` keyed random pairs.  Acceptance stays the exact rule of
+ * accelerate.cpp:663-689.  Call before hea_session_init/step. */
+int hea_session_guide(hea_session s, int32_t every, int32_t candidates, double tau);
 /* Runs `n_iter` global iterations asynchronously on the session stream. */
 int hea_session_step(hea_session s, int32_t n_iter);
 /* Measurement helper with no counterpart in the reference: runs n_iter
@@ -146,6 +155,48 @@ int hea_session_read(hea_session s, uint8_t* out_lattices, double* out_fitness, 
 /* Population best fitness after the last completed iteration. */
 int hea_session_best(hea_session s, double* best);
 int hea_session_destroy(hea_session s);
+
+/* Batched search with no counterpart in the reference (which runs one
+ * problem per call, accelerate.cpp:713-763): n_probs independent problems on
+ * one device, each with its own population of `solutions` lattices, all
+ * searched by one persistent local-search launch per phase.  Problem p
+ * draws from seed `seed + p`, so its population equals
+ * hea_hcs_run(probs[p], ..., seed + p, ...).  Problems must share the
+ * device, the shell count and have no self neighbours; parallel_task and
+ * converge_depth are common.  The problems must outlive the batch. */
+int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t solutions,
+                     int32_t parallel_task, int32_t converge_depth, double thr, uint64_t seed,
+                     hea_batch* out);
+/* Sets the number of persistent workgroups (launch shape chosen by the
+ * autotuner; results do not depend on it). */
+int hea_batch_grid(hea_batch b, int32_t grid);
+/* Guided proposals for every problem of the batch (see hea_session_guide). */
+int hea_batch_guide(hea_batch b, int32_t every, int32_t candidates, double tau);
+int hea_batch_init(hea_batch b);
+int hea_batch_step(hea_batch b, int32_t n_iter);
+int hea_batch_sync(hea_batch b);
+/* Population (sorted) and counters of problem p. */
+int hea_batch_read(hea_batch b, int32_t p, uint8_t* out_lattices, double* out_fitness,
+                   hea_stats* stats);
+int hea_batch_destroy(hea_batch b);
+
+/* Differentiable relaxation, no counterpart in the reference (which searches
+ * only discrete lattices).  With p = softmax(logits) per site, the soft pair
+ * counts n[s][a][b] = sum_i sum_{j in N_s(i)} p[i][a] p[j][b] equal the integer
+ * pair counts when p is one-hot.  All pointers are device memory; dtype 0 =
+ * float32, 1 = float64 for logits, p [N][K], q [S][N][K] (neighbour sums
+ * kept for the backward) and grad [N][K]; n and dn are float64 [S][K][K].
+ * Asynchronous on `stream`.  Calls on one problem must not overlap. */
+int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
+                      void* d_n, void* stream);
+/* Neighbour-gather adjoint: d loss / d logits from dn = d loss / d n. */
+int hea_relax_backward(hea_problem p, int32_t dtype, const void* d_p, const void* d_q,
+                       const void* d_dn, void* d_grad, void* stream);
+/* Swap guidance at discrete lattices: u[l][k][b] (float32) = first-order
+ * loss change of turning site k of lattice l into species b, from
+ * dn[l] = d loss / d n (float64 [L][S][K][K]). */
+int hea_site_gain(hea_problem p, int32_t L, const void* d_lattices, const void* d_dn, void* d_u,
+                  void* stream);
 
 #ifdef __cplusplus
 }

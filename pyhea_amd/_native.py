@@ -31,12 +31,24 @@ EXPORTS = (
     "hea_generate",
     "hea_session_create",
     "hea_session_init",
+    "hea_session_guide",
     "hea_session_step",
     "hea_session_step_timed",
     "hea_session_sync",
     "hea_session_read",
     "hea_session_best",
     "hea_session_destroy",
+    "hea_batch_create",
+    "hea_batch_grid",
+    "hea_batch_guide",
+    "hea_batch_init",
+    "hea_batch_step",
+    "hea_batch_sync",
+    "hea_batch_read",
+    "hea_batch_destroy",
+    "hea_relax_forward",
+    "hea_relax_backward",
+    "hea_site_gain",
 )
 
 
@@ -95,6 +107,7 @@ def _declare(lib: C.CDLL) -> None:
         "hea_generate": (C.c_int, [P, I32, U64, U32, U32, u8p, dp]),
         "hea_session_create": (C.c_int, [P, I32, I32, I32, D, U64, C.POINTER(P)]),
         "hea_session_init": (C.c_int, [P]),
+        "hea_session_guide": (C.c_int, [P, I32, I32, D]),
         "hea_session_step": (C.c_int, [P, I32]),
         "hea_session_step_timed": (C.c_int, [P, I32, C.POINTER(C.c_float),
                                              C.POINTER(C.c_int64)]),
@@ -102,6 +115,17 @@ def _declare(lib: C.CDLL) -> None:
         "hea_session_read": (C.c_int, [P, u8p, dp, C.POINTER(HeaStats)]),
         "hea_session_best": (C.c_int, [P, dp]),
         "hea_session_destroy": (C.c_int, [P]),
+        "hea_batch_create": (C.c_int, [C.POINTER(P), I32, I32, I32, I32, D, U64, C.POINTER(P)]),
+        "hea_batch_grid": (C.c_int, [P, I32]),
+        "hea_batch_guide": (C.c_int, [P, I32, I32, D]),
+        "hea_batch_init": (C.c_int, [P]),
+        "hea_batch_step": (C.c_int, [P, I32]),
+        "hea_batch_sync": (C.c_int, [P]),
+        "hea_batch_read": (C.c_int, [P, I32, u8p, dp, C.POINTER(HeaStats)]),
+        "hea_batch_destroy": (C.c_int, [P]),
+        "hea_relax_forward": (C.c_int, [P, I32, P, P, P, P, P]),
+        "hea_relax_backward": (C.c_int, [P, I32, P, P, P, P, P]),
+        "hea_site_gain": (C.c_int, [P, I32, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -179,6 +179,13 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Guided-proposal settings of a local-search launch (gain == nullptr: plain).
+struct GuideCfg {
+  float* gain = nullptr;
+  int every = 0, cand = 0;
+  double tau = 0.0;
+};
+
 }  // namespace
 
 struct hea_problem_s {
@@ -194,6 +201,7 @@ struct hea_problem_s {
   DevBuf<int32_t> d_counts, d_perm;
   DevBuf<int64_t> d_stats;
   DevBuf<uint32_t> d_iter;
+  DevBuf<double> d_relax_part;  // relaxation chunk partials
 };
 
 struct hea_session_s {
@@ -213,6 +221,8 @@ struct hea_session_s {
   DevBuf<uint32_t> iter;
   DevBuf<unsigned long long> clk;  // [0] start, [1] first fitness <= thr
   int iters_done = 0;
+  DevBuf<float> gain;  // guided mode
+  GuideCfg guide;
 };
 
 namespace {
@@ -322,10 +332,27 @@ int launch_merge(const uint8_t* A, const double* fa, int na, const uint8_t* B, c
   return HEA_OK;
 }
 
+// Problem scalars of a local-search launch.
+void fill_problem_args(hea_problem p, HeaMcArgs& a) {
+  const HeaConst& c = p->hc;
+  a.pc = p->d_const.p;
+  a.nbr = p->d_nbr.p;
+  a.N = c.N;
+  a.K = c.K;
+  a.S = c.S;
+  a.stride = c.stride;
+  for (int s = 0; s < HEA_MAX_S; ++s) {
+    a.Z[s] = c.Z[s];
+    a.off[s] = c.off[s];
+    a.delta[s] = c.delta[s];
+    a.weight[s] = c.weight[s];
+  }
+}
+
 int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, int64_t* stats,
               const uint32_t* iter_ptr, unsigned long long* first_hit, int L, int task,
               int depth, uint64_t seed, uint32_t phase, double thr, size_t lds, bool sp_lds,
-              hipStream_t st) {
+              hipStream_t st, const GuideCfg& gd = GuideCfg()) {
   HeaMcArgs a;
   a.pc = p->d_const.p;
   a.nbr = p->d_nbr.p;
@@ -342,6 +369,10 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.tp = hea_mc_tp(task, p->hc.S);
   a.phase = phase;
   a.thr = thr;
+  a.gain = gd.gain;
+  a.g_every = gd.every;
+  a.g_cand = gd.cand;
+  a.g_tau = gd.tau;
   const HeaConst& c = p->hc;
   a.N = c.N;
   a.K = c.K;
@@ -353,7 +384,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
     a.delta[s] = c.delta[s];
     a.weight[s] = c.weight[s];
   }
-  HEA_HIP(hea_launch_local_mc(a, p->hc.K, sp_lds, lds, st));
+  HEA_HIP(hea_launch_local_mc(a, p->hc.K, sp_lds, p->hc.self_nbr != 0, lds, st));
   return HEA_OK;
 }
 
@@ -368,7 +399,8 @@ int check_run_args(hea_problem p, int L, int task, int depth) {
 
 // LDS bytes of a local-search launch; species live in LDS when they fit.
 int mc_lds(hea_problem p, int task, size_t* lds, bool* sp_lds) {
-  const int km = hea_mc_kmax(p->hc.K);
+  const bool self = p->hc.self_nbr != 0;
+  const int km = hea_mc_kmax(p->hc.K, self);
   const int tp = hea_mc_tp(task, p->hc.S);
   *sp_lds = !p->force_hbm_species;
   *lds = hea_mc_lds_bytes(km, task, tp, p->hc.N, *sp_lds);
@@ -377,7 +409,7 @@ int mc_lds(hea_problem p, int task, size_t* lds, bool* sp_lds) {
     *lds = hea_mc_lds_bytes(km, task, tp, p->hc.N, false);
     if ((int64_t)*lds > p->max_lds) return fail(HEA_EUNSUPPORTED, "parallel_task too large for LDS");
   }
-  HEA_HIP(hea_prepare_local_mc(p->hc.K, *sp_lds, *lds));
+  HEA_HIP(hea_prepare_local_mc(p->hc.K, *sp_lds, self, *lds));
   return HEA_OK;
 }
 
@@ -659,7 +691,8 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
     HEA_HIP(hipEventRecord(hk->ev[0], st));
   }
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
-                      s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, s->sp_lds, st)))
+                      s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, s->sp_lds, st,
+                      s->guide)))
     return rc;
   if (hk) {
     HEA_HIP(hipEventRecord(hk->ev[1], st));
@@ -676,7 +709,8 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
   if ((rc = launch_recount(p, s->B.p, s->counts.p, L, st))) return rc;
   if (hk) HEA_HIP(hipEventRecord(hk->ev[2], st));
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
-                      s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, s->sp_lds, st)))
+                      s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, s->sp_lds, st,
+                      s->guide)))
     return rc;
   if (hk) {
     HEA_HIP(hipEventRecord(hk->ev[3], st));
@@ -732,6 +766,25 @@ int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, do
     return fail(HEA_EHIP, hipGetErrorString(e));
   }
   *out = s;
+  return HEA_OK;
+}
+
+int hea_session_guide(hea_session s, int32_t every, int32_t candidates, double tau) {
+  if (!s || every < 1 || candidates < 1 || !(tau > 0.0))
+    return fail(HEA_EINVAL, "guide needs every >= 1, candidates >= 1, tau > 0");
+  HEA_HIP(hipSetDevice(s->p->device));
+  const size_t n = (size_t)s->L * s->p->hc.N * s->p->hc.K;
+  if (s->gain.n < n) HEA_HIP(s->gain.alloc(n));
+  s->guide.gain = s->gain.p;
+  s->guide.every = every;
+  s->guide.cand = candidates;
+  s->guide.tau = tau;
+  if (s->exec) {  // the graph bakes the launch arguments in: capture again
+    (void)hipGraphExecDestroy(s->exec);
+    (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+  }
   return HEA_OK;
 }
 
@@ -871,6 +924,386 @@ int hea_hcs_run(hea_problem p, int32_t L, int32_t total_iter, int32_t task, int3
     rc = hea_session_read(s, out_lat, out_fit, stats);
   hea_session_destroy(s);
   return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------- batch --
+// Many independent problems searched together.  Problem p keeps its own
+// population (A: current, B: searched, C: sorted scratch) and draws from seed
+// `seed + p`, so its result equals hea_hcs_run(problem p, seed + p).  The
+// cheap phases are launched per problem; both local searches run as one
+// persistent launch over all (problem, lattice) items, longest first.  One
+// global iteration is captured as one HIP graph.
+
+struct hea_batch_s {
+  std::vector<hea_problem> probs;
+  int L = 0, task = 0, depth = 0, tp = 0, S = 0, kmax = 0, threads = 0, grid = 0;
+  double thr = 0.0;
+  uint64_t seed = 0;
+  bool sp_lds = true;
+  size_t lds = 0;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<size_t> lat_off, cnt_off;  // element offsets per problem
+  DevBuf<uint8_t> A, B, C;
+  DevBuf<double> fA, fB, fC;
+  DevBuf<int32_t> counts, perm;
+  DevBuf<int64_t> stats;
+  DevBuf<uint32_t> iter, queue;
+  DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
+  DevBuf<HeaMcArgs> margs;         // per-problem fields of the searched population B
+  DevBuf<int2> items;
+  DevBuf<float> gain;              // guided mode, [sum_p L N_p K_p]
+  GuideCfg guide;
+};
+
+extern "C" int hea_batch_grid(hea_batch b, int32_t grid);  // autotuner hook (below)
+
+static int batch_enqueue_iteration(hea_batch b) {
+  const int L = b->L, P = (int)b->probs.size();
+  hipStream_t st = b->stream;
+  int rc;
+  for (int q = 0; q < P; ++q) {
+    hea_problem p = b->probs[q];
+    const int N = p->hc.N;
+    dim3 g((N + 255) / 256, L);
+    hipLaunchKernelGGL(k_permute, g, dim3(256), 0, st, p->d_const.p, b->B.p + b->lat_off[q],
+                       b->seed + (uint64_t)q, b->iter.p, (uint32_t)HEA_PHASE_NEW, 0);
+    if ((rc = launch_recount(p, b->B.p + b->lat_off[q], b->counts.p + b->cnt_off[q], L, st)))
+      return rc;
+    if ((rc = launch_fitness(p, b->counts.p + b->cnt_off[q], b->fB.p + (size_t)q * L, L, st)))
+      return rc;
+  }
+  HeaMcBatchArgs ba;
+  ba.probs = b->margs.p;
+  ba.items = b->items.p;
+  ba.n_items = P * L;
+  ba.queue = b->queue.p;
+  ba.run.task = b->task;
+  ba.run.depth = b->depth;
+  ba.run.tp = b->tp;
+  ba.run.thr = b->thr;
+  ba.run.iter_ptr = b->iter.p;
+  ba.run.seed = b->seed;
+  ba.run.g_every = b->guide.every;
+  ba.run.g_cand = b->guide.cand;
+  ba.run.g_tau = b->guide.tau;
+  for (int phase = HEA_PHASE_LS1; phase <= HEA_PHASE_LS2; ++phase) {
+    ba.run.phase = (uint32_t)phase;
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, b->queue.p, 0u);
+    HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->grid, b->threads, b->lds, st));
+    for (int q = 0; q < P; ++q) {
+      hea_problem p = b->probs[q];
+      const int N = p->hc.N;
+      uint8_t* A = b->A.p + b->lat_off[q];
+      uint8_t* B = b->B.p + b->lat_off[q];
+      uint8_t* C = b->C.p + b->lat_off[q];
+      double* fA = b->fA.p + (size_t)q * L;
+      double* fB = b->fB.p + (size_t)q * L;
+      double* fC = b->fC.p + (size_t)q * L;
+      int32_t* perm = b->perm.p + (size_t)q * 2 * L;
+      if (phase == HEA_PHASE_LS1) {
+        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, C, fC, N, st))) return rc;
+        if ((rc = launch_merge(A, fA, L, C, fC, L, L, perm, B, fB, N, st))) return rc;
+        if ((rc = launch_recount(p, B, b->counts.p + b->cnt_off[q], L, st))) return rc;
+      } else {
+        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, A, fA, N, st))) return rc;
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_iter_add, dim3(1), dim3(1), 0, st, b->iter.p, 1u);
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+extern "C" {
+
+int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32_t task,
+                     int32_t depth, double thr, uint64_t seed, hea_batch* out) {
+  if (!out) return fail(HEA_EINVAL, "null output handle");
+  *out = nullptr;
+  if (!probs || n_probs < 1) return fail(HEA_EINVAL, "need at least one problem");
+  int rc;
+  for (int q = 0; q < n_probs; ++q)
+    if ((rc = check_run_args(probs[q], L, task, depth))) return rc;
+  const int dev = probs[0]->device, S = probs[0]->hc.S;
+  int kmax_need = 0, maxN = 0;
+  for (int q = 0; q < n_probs; ++q) {
+    if (probs[q]->device != dev) return fail(HEA_EINVAL, "batch problems must share a device");
+    if (probs[q]->hc.S != S) return fail(HEA_EINVAL, "batch problems must share the shell count");
+    if (probs[q]->hc.self_nbr)
+      return fail(HEA_EUNSUPPORTED, "batched search needs cells without self neighbours");
+    kmax_need = std::max(kmax_need, probs[q]->hc.K);
+    maxN = std::max(maxN, probs[q]->hc.N);
+  }
+  HEA_HIP(hipSetDevice(dev));
+  hea_batch b = new hea_batch_s();
+  b->probs.assign(probs, probs + n_probs);
+  b->L = L;
+  b->task = task;
+  b->depth = depth;
+  b->thr = thr;
+  b->seed = seed;
+  b->S = S;
+  b->kmax = kmax_need <= 6 ? 6 : 8;
+  b->tp = hea_mc_tp(task, S);
+  b->threads = S * b->tp;
+  b->sp_lds = !probs[0]->force_hbm_species;
+  b->lds = hea_mc_lds_bytes(b->kmax, task, b->tp, maxN, b->sp_lds);
+  if ((int64_t)b->lds > probs[0]->max_lds) {
+    b->sp_lds = false;
+    b->lds = hea_mc_lds_bytes(b->kmax, task, b->tp, maxN, false);
+  }
+  size_t nl = 0, nc = 0;
+  std::vector<std::pair<double, int2>> work;
+  for (int q = 0; q < n_probs; ++q) {
+    const HeaConst& c = probs[q]->hc;
+    b->lat_off.push_back(nl);
+    b->cnt_off.push_back(nc);
+    nl += (size_t)L * c.N;
+    nc += (size_t)L * c.S * c.K * c.K;
+    int sz = 0;
+    for (int s = 0; s < c.S; ++s) sz += c.Z[s];
+    const double cost = (double)c.N * (sz + c.K * c.K);  // rounds ~ N, round ~ sumZ + K^2
+    for (int l = 0; l < L; ++l) work.push_back({cost, make_int2(q, l)});
+  }
+  std::stable_sort(work.begin(), work.end(),
+                   [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
+                     return x.first > y.first;
+                   });
+  std::vector<int2> items;
+  for (auto& w : work) items.push_back(w.second);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = b->A.alloc(nl);
+  if (e == hipSuccess) e = b->B.alloc(nl);
+  if (e == hipSuccess) e = b->C.alloc(nl);
+  if (e == hipSuccess) e = b->fA.alloc((size_t)n_probs * L);
+  if (e == hipSuccess) e = b->fB.alloc((size_t)n_probs * L);
+  if (e == hipSuccess) e = b->fC.alloc((size_t)n_probs * L);
+  if (e == hipSuccess) e = b->counts.alloc(nc);
+  if (e == hipSuccess) e = b->perm.alloc((size_t)n_probs * 2 * L);
+  if (e == hipSuccess) e = b->stats.alloc((size_t)n_probs * 4 * L);
+  if (e == hipSuccess) e = b->iter.alloc(1);
+  if (e == hipSuccess) e = b->queue.alloc(1);
+  if (e == hipSuccess) e = b->clk.alloc((size_t)1 + n_probs);
+  if (e == hipSuccess) e = b->margs.alloc(n_probs);
+  if (e == hipSuccess) e = b->items.alloc(items.size());
+  if (e != hipSuccess) {
+    delete b;
+    return fail(HEA_ENOMEM, std::string("batch allocation: ") + hipGetErrorString(e));
+  }
+  std::vector<HeaMcArgs> ma(n_probs);
+  for (int q = 0; q < n_probs; ++q) {
+    HeaMcArgs& a = ma[q];
+    memset(&a, 0, sizeof(a));
+    fill_problem_args(probs[q], a);
+    a.lattices = b->B.p + b->lat_off[q];
+    a.fitness = b->fB.p + (size_t)q * L;
+    a.counts = b->counts.p + b->cnt_off[q];
+    a.stats = b->stats.p + (size_t)q * 4 * L;
+    a.first_hit = b->clk.p + 1 + q;
+    a.L = L;
+  }
+  e = hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * n_probs, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(HEA_EHIP, hipGetErrorString(e));
+  }
+  int cus = 0;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid = std::min((int)items.size(), std::max(1, cus) * 2);
+  *out = b;
+  return HEA_OK;
+}
+
+int hea_batch_grid(hea_batch b, int32_t grid) {
+  if (!b || grid < 1) return fail(HEA_EINVAL, "bad arguments");
+  b->grid = std::min(grid, (int32_t)(b->probs.size() * b->L));
+  if (b->exec) {  // the graph bakes the launch shape in: capture again
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  return HEA_OK;
+}
+
+int hea_batch_guide(hea_batch b, int32_t every, int32_t candidates, double tau) {
+  if (!b || every < 1 || candidates < 1 || !(tau > 0.0))
+    return fail(HEA_EINVAL, "guide needs every >= 1, candidates >= 1, tau > 0");
+  HEA_HIP(hipSetDevice(b->probs[0]->device));
+  const int P = (int)b->probs.size();
+  size_t n = 0;
+  std::vector<size_t> off(P);
+  for (int q = 0; q < P; ++q) {
+    off[q] = n;
+    n += (size_t)b->L * b->probs[q]->hc.N * b->probs[q]->hc.K;
+  }
+  if (b->gain.n < n) HEA_HIP(b->gain.alloc(n));
+  std::vector<HeaMcArgs> ma(P);
+  HEA_HIP(hipMemcpy(ma.data(), b->margs.p, sizeof(HeaMcArgs) * P, hipMemcpyDeviceToHost));
+  for (int q = 0; q < P; ++q) ma[q].gain = b->gain.p + off[q];
+  HEA_HIP(hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * P, hipMemcpyHostToDevice));
+  b->guide.gain = b->gain.p;
+  b->guide.every = every;
+  b->guide.cand = candidates;
+  b->guide.tau = tau;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  return HEA_OK;
+}
+
+int hea_batch_init(hea_batch b) {
+  if (!b) return fail(HEA_EINVAL, "null batch");
+  HEA_HIP(hipSetDevice(b->probs[0]->device));
+  const int L = b->L, P = (int)b->probs.size();
+  hipStream_t st = b->stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st,
+                     reinterpret_cast<int32_t*>(b->stats.p), (size_t)8 * L * P);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, b->iter.p, 0u);
+  for (int q = 0; q < P; ++q)
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, b->clk.p + 1 + q, ~0ull);
+  hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, st, b->clk.p);
+  int rc;
+  for (int q = 0; q < P; ++q) {
+    hea_problem p = b->probs[q];
+    dim3 g((p->hc.N + 255) / 256, L);
+    hipLaunchKernelGGL(k_permute, g, dim3(256), 0, st, p->d_const.p, b->A.p + b->lat_off[q],
+                       b->seed + (uint64_t)q, b->iter.p, (uint32_t)HEA_PHASE_INIT, 0);
+    if ((rc = launch_recount(p, b->A.p + b->lat_off[q], b->counts.p + b->cnt_off[q], L, st)))
+      return rc;
+    if ((rc = launch_fitness(p, b->counts.p + b->cnt_off[q], b->fA.p + (size_t)q * L, L, st)))
+      return rc;
+  }
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+int hea_batch_step(hea_batch b, int32_t n_iter) {
+  if (!b || n_iter < 0) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(b->probs[0]->device));
+  if (!b->exec) {
+    HEA_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue_iteration(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(HEA_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    HEA_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < n_iter; ++i) HEA_HIP(hipGraphLaunch(b->exec, b->stream));
+  return HEA_OK;
+}
+
+int hea_batch_sync(hea_batch b) {
+  if (!b) return fail(HEA_EINVAL, "null batch");
+  HEA_HIP(hipSetDevice(b->probs[0]->device));
+  HEA_HIP(hipStreamSynchronize(b->stream));
+  return HEA_OK;
+}
+
+int hea_batch_read(hea_batch b, int32_t q, uint8_t* out_lat, double* out_fit, hea_stats* stats) {
+  if (!b || q < 0 || q >= (int)b->probs.size()) return fail(HEA_EINVAL, "bad arguments");
+  hea_problem p = b->probs[q];
+  HEA_HIP(hipSetDevice(p->device));
+  HEA_HIP(hipStreamSynchronize(b->stream));
+  const int L = b->L;
+  if (out_lat)
+    HEA_HIP(hipMemcpy(out_lat, b->A.p + b->lat_off[q], (size_t)L * p->hc.N,
+                      hipMemcpyDeviceToHost));
+  if (out_fit)
+    HEA_HIP(hipMemcpy(out_fit, b->fA.p + (size_t)q * L, sizeof(double) * L,
+                      hipMemcpyDeviceToHost));
+  if (stats) {
+    std::vector<int64_t> st((size_t)4 * L);
+    HEA_HIP(hipMemcpy(st.data(), b->stats.p + (size_t)q * 4 * L, sizeof(int64_t) * st.size(),
+                      hipMemcpyDeviceToHost));
+    sum_stats(st, L, stats);
+    unsigned long long c0 = 0, c1 = 0;
+    HEA_HIP(hipMemcpy(&c0, b->clk.p, sizeof(c0), hipMemcpyDeviceToHost));
+    HEA_HIP(hipMemcpy(&c1, b->clk.p + 1 + q, sizeof(c1), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HEA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, p->device));
+    if (c1 != ~0ull && khz > 0) stats->first_hit_s = (double)(c1 - c0) / (1000.0 * khz);
+  }
+  return HEA_OK;
+}
+
+int hea_batch_destroy(hea_batch b) {
+  if (!b) return HEA_OK;
+  (void)hipSetDevice(b->probs[0]->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return HEA_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- relaxation --
+hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
+                                  int dtype, const void* th, void* p, void* Q, double* part,
+                                  double* n, hipStream_t st);
+hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
+                                   const void* p, const void* Q, const double* dn, void* grad,
+                                   hipStream_t st);
+hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
+                              const uint8_t* lattices, int L, const double* dn, float* u,
+                              hipStream_t st);
+int hea_relax_chunks(int n_sites);
+
+extern "C" {
+
+int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
+                      void* d_n, void* stream) {
+  if (!p || !d_logits || !d_p || !d_q || !d_n || (dtype != 0 && dtype != 1))
+    return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(p->device));
+  const HeaConst& c = p->hc;
+  const size_t need = (size_t)hea_relax_chunks(c.N) * c.S * c.K * c.K;
+  if (p->d_relax_part.n < need) HEA_HIP(p->d_relax_part.alloc(need));
+  HEA_HIP(hea_relax_forward_impl(p->d_const.p, c, p->d_nbr.p, dtype, d_logits, d_p, d_q,
+                                 p->d_relax_part.p, static_cast<double*>(d_n),
+                                 static_cast<hipStream_t>(stream)));
+  return HEA_OK;
+}
+
+int hea_relax_backward(hea_problem p, int32_t dtype, const void* d_p, const void* d_q,
+                       const void* d_dn, void* d_grad, void* stream) {
+  if (!p || !d_p || !d_q || !d_dn || !d_grad || (dtype != 0 && dtype != 1))
+    return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(p->device));
+  HEA_HIP(hea_relax_backward_impl(p->d_const.p, p->hc, dtype, d_p, d_q,
+                                  static_cast<const double*>(d_dn), d_grad,
+                                  static_cast<hipStream_t>(stream)));
+  return HEA_OK;
+}
+
+int hea_site_gain(hea_problem p, int32_t L, const void* d_lattices, const void* d_dn, void* d_u,
+                  void* stream) {
+  if (!p || L < 1 || !d_lattices || !d_dn || !d_u) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(p->device));
+  HEA_HIP(hea_site_gain_impl(p->d_const.p, p->hc, p->d_nbr.p,
+                             static_cast<const uint8_t*>(d_lattices), L,
+                             static_cast<const double*>(d_dn), static_cast<float*>(d_u),
+                             static_cast<hipStream_t>(stream)));
+  return HEA_OK;
 }
 
 }  // extern "C"

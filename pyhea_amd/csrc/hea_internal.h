@@ -52,10 +52,27 @@ struct HeaMcArgs {
   int32_t L, task, depth, tp;   // tp: proposals per pass (lanes per shell)
   uint32_t phase;
   double thr;
+  // guided proposals (gain == nullptr: plain keyed stream)
+  float* gain;                  // [L][N][K] site gains, refreshed every g_every rounds
+  int32_t g_every, g_cand;      // refresh period (rounds), candidates per proposal
+  double g_tau;                 // softmax temperature in units of the gain scale
   // problem scalars, copied from HeaConst so they live in the kernarg segment
   int32_t N, K, S, stride;
   int32_t Z[HEA_MAX_S], off[HEA_MAX_S];
   double delta[HEA_MAX_S], weight[HEA_MAX_S];
+};
+
+// One launch of the persistent local search over many problems.  probs[p]
+// holds problem p's population fields (pc, nbr, lattices, fitness, counts,
+// stats, first_hit, L and the problem scalars); run holds the shared run
+// fields (task, depth, tp, phase, thr, iter_ptr, seed).  items[i] =
+// (problem, lattice), processed in order.
+struct HeaMcBatchArgs {
+  const HeaMcArgs* probs;
+  const int2* items;
+  int32_t n_items;
+  uint32_t* queue;  // zeroed before each launch
+  HeaMcArgs run;
 };
 
 // R_s(k) from the binade segments (exact: the host verified every k).
@@ -67,8 +84,11 @@ __device__ __forceinline__ double hea_r_of_count(const HeaConst& P, int s, int64
 
 // Launchers implemented in hea_mc.hip.
 size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds);
-int hea_mc_kmax(int K);
+int hea_mc_kmax(int K, bool self);
 int hea_mc_tp(int task, int S);
-hipError_t hea_prepare_local_mc(int K, bool sp_lds, size_t lds);
-hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, size_t lds,
+hipError_t hea_prepare_local_mc(int K, bool sp_lds, bool self, size_t lds);
+hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self, size_t lds,
                                hipStream_t st);
+hipError_t hea_prepare_local_mc_batch(int kmax, bool sp_lds, size_t lds);
+hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_lds, int grid,
+                                     int threads, size_t lds, hipStream_t st);

@@ -55,6 +55,7 @@ struct McShared {
   int wj[kMaxWaves];
   uint32_t next_pos;
   int pad;
+  double gscale;  // guided mode: scale of the site gains
 };
 
 // Species histogram of one shell walk, packed one field per species:
@@ -117,7 +118,7 @@ __device__ __forceinline__ double replay_ab(double v, double delta, const int32_
 
 // Species histogram of one neighbour list: M += 1<<(W*species); cs counts
 // `self`, co counts `other`.  Rows are padded to a multiple of 4 entries.
-template <int KMAX>
+template <int KMAX, bool SELF>
 __device__ __forceinline__ void walk(const int32_t* __restrict__ row, int z,
                                      const uint8_t* __restrict__ sp, int self, int other,
                                      typename Pack<KMAX>::T& M, int& cs, int& co) {
@@ -132,7 +133,7 @@ __device__ __forceinline__ void walk(const int32_t* __restrict__ row, int z,
       if (q + t < z) {
         const int n = nb[t];
         M += Pack<KMAX>::one((int)sp[n]);
-        cs += (n == self);
+        if (SELF) cs += (n == self);
         co += (n == other);
       }
     }
@@ -147,13 +148,14 @@ struct Lds {
   double* E0;    // [S][KMAX][KMAX] (1 - G/d - t)^2 of prev_gamma
   double* R;     // [S][KMAX][KMAX] safe step or 0
   double* tgt;   // [S][KMAX][KMAX] target alpha
+  double* H;     // [S][KMAX][KMAX] guided mode: dF2/dn + its transpose
   double* conc;  // [KMAX] fl(count/N)
   double* esum;  // [S][TP] per-shell partial sums
   int2* prop;    // [task]
   McShared* sh;
   uint8_t* sp;   // [N] when resident
   static __host__ __device__ size_t bytes(int task, int tp, int n, bool sp_lds) {
-    size_t b = sizeof(double) * (4 * TAB + KMAX + HEA_MAX_S * (size_t)tp);
+    size_t b = sizeof(double) * (5 * TAB + KMAX + HEA_MAX_S * (size_t)tp);
     b += sizeof(int2) * (size_t)task + sizeof(McShared);
     if (sp_lds) b += ((size_t)n + 15) & ~(size_t)15;
     return b;
@@ -163,7 +165,8 @@ struct Lds {
     E0 = G + TAB;
     R = E0 + TAB;
     tgt = R + TAB;
-    conc = tgt + TAB;
+    H = tgt + TAB;
+    conc = H + TAB;
     esum = conc + KMAX;
     prop = reinterpret_cast<int2*>(esum + HEA_MAX_S * tp);
     sh = reinterpret_cast<McShared*>(prop + task);
@@ -186,12 +189,12 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
 // Shell s of the incremental fitness of swapping a1 <-> a2 (species a != b)
 // against prev_gamma.  Returns the shell's partial sum and fills the swapped
 // gamma rows a (nA) and b (nB) of that shell.
-template <int KMAX>
+template <int KMAX, bool EXACT, bool SELF>
 __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>& T,
                                              const uint8_t* __restrict__ sp, int s, int a1,
                                              int a2, double (&nA)[KMAX], double (&nB)[KMAX]) {
   using PK = Pack<KMAX>;
-  const int K = A.K;
+  const int K = EXACT ? KMAX : A.K;
   const int z = A.Z[s];
   const double delta = A.delta[s];
   const int a = sp[a1], b = sp[a2];
@@ -199,8 +202,8 @@ __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>
   const int32_t* r2 = A.nbr + (size_t)a2 * A.stride + A.off[s];
   typename PK::T MA, MB;
   int c11, c12, c22, c21;
-  walk<KMAX>(r1, z, sp, a1, a2, MA, c11, c12);
-  walk<KMAX>(r2, z, sp, a2, a1, MB, c22, c21);
+  walk<KMAX, SELF>(r1, z, sp, a1, a2, MA, c11, c12);
+  walk<KMAX, SELF>(r2, z, sp, a2, a1, MB, c22, c21);
   const int maA = PK::fld(MA, a), maB = PK::fld(MA, b);
   const int mbA = PK::fld(MB, a), mbB = PK::fld(MB, b);
 
@@ -221,7 +224,7 @@ __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>
   for (int x = 0; x < KMAX; ++x) {
     ma[x] = PK::fld(MA, x);
     mb[x] = PK::fld(MB, x);
-    if (x < K) {
+    if (EXACT || x < K) {
       const int eA = T.idx(s, a, x), eB = T.idx(s, b, x);
       const double gA = T.G[eA], rA = T.R[eA];
       const double gB = T.G[eB], rB = T.R[eB];
@@ -247,7 +250,7 @@ __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>
   double sA[KMAX], sB[KMAX];
 #pragma unroll
   for (int x = 0; x < KMAX; ++x) {
-    if (x < K) {
+    if (EXACT || x < K) {
       nA[x] = (x == a) ? vAA : (x == b) ? vAB : nA[x];
       nB[x] = (x == b) ? vBB : (x == a) ? vAB : nB[x];
       sA[x] = 1.0 - nA[x] / (conc_a * T.conc[x]);
@@ -261,11 +264,11 @@ __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>
   double es = 0.0;
 #pragma unroll
   for (int i = 0; i < KMAX; ++i) {
-    if (i < K) {
+    if (EXACT || i < K) {
       const bool ia = (i == a), ib = (i == b);
 #pragma unroll
       for (int j = 0; j < KMAX; ++j) {
-        if (j < K) {
+        if (EXACT || j < K) {
           const bool ja = (j == a), jb = (j == b);
           const double sro = ia ? sA[j] : ib ? sB[j] : ja ? sA[i] : sB[i];
           const double df = sro - tg[i * KMAX + j];
@@ -279,16 +282,66 @@ __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>
   return es;
 }
 
+// Guided mode: site gains u[k][c] = g[k][c] - g[k][sp[k]] of lattice l at
+// its current gamma, with g[k][c] = sum_s sum_b H_s[c][b] cnt_s[k][b] and
+// H_s = dF2/dn_s + (dF2/dn_s)^T, dF2/dn_sab = -2 w_s (alpha_sab - t_sab) /
+// (Z_s N c_a c_b) (the closed-form derivative of the reference fitness
+// squared, accelerate.cpp:556-569, at gamma = n / (Z_s N)).
+template <int KMAX>
+__device__ void refresh_gain(const HeaMcArgs& A, const Lds<KMAX>& T, const uint8_t* sp, int l) {
+  const int K = A.K, S = A.S, N = A.N, nthr = blockDim.x, tid = threadIdx.x;
+  for (int e = tid; e < S * K * K; e += nthr) {
+    const int s = e / (K * K), c = (e / K) % K, b = e % K;
+    const double zn = (double)A.Z[s] * N;
+    const double ccb = T.conc[c] * T.conc[b];
+    const double dcb = -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, c, b)] / ccb) - T.tgt[T.idx(s, c, b)]) /
+                       (zn * ccb);
+    const double dbc = -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, b, c)] / ccb) - T.tgt[T.idx(s, b, c)]) /
+                       (zn * ccb);
+    T.H[T.idx(s, c, b)] = dcb + dbc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double sc = 0.0;
+    for (int s = 0; s < S; ++s) {
+      double m = 0.0;
+      for (int c = 0; c < K; ++c)
+        for (int b = 0; b < K; ++b) m = fmax(m, fabs(T.H[T.idx(s, c, b)]));
+      sc += A.Z[s] * m;
+    }
+    T.sh->gscale = sc > 0.0 ? sc : 1.0;
+  }
+  float* u = A.gain + (size_t)l * N * K;
+  for (int k = tid; k < N; k += nthr) {
+    const int32_t* row = A.nbr + (size_t)k * A.stride;
+    double g[KMAX];
+    for (int c = 0; c < KMAX; ++c) g[c] = 0.0;
+    for (int s = 0; s < S; ++s) {
+      uint64_t m = 0;  // 8-bit species counts of this shell
+      for (int q = 0; q < A.Z[s]; ++q) m += 1ull << (8 * sp[row[A.off[s] + q]]);
+      for (int b = 0; b < K; ++b) {
+        const double cb = (double)((m >> (8 * b)) & 255u);
+        if (cb != 0.0)
+          for (int c = 0; c < K; ++c) g[c] += T.H[T.idx(s, c, b)] * cb;
+      }
+    }
+    const int a = sp[k];
+    double ga = 0.0;
+    for (int c = 0; c < K; ++c) ga = (c == a) ? g[c] : ga;
+    for (int c = 0; c < K; ++c) u[(size_t)k * K + c] = (float)(g[c] - ga);
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned long long f2,
                                          int j2) {
   return (f1 < f2) || (f1 == f2 && j1 < j2);
 }
 
-template <int KMAX, bool SP_LDS>
-__global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// Local search of lattice l of the population described by A.
+template <int KMAX, bool SP_LDS, bool EXACT, bool SELF>
+__device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned char* smem) {
   const HeaConst& P = *A.pc;
-  const int l = blockIdx.x;
   const int nthr = blockDim.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int N = A.N, K = A.K, S = A.S;
@@ -332,9 +385,43 @@ __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
   bool hit_thr = false;
   double nA[KMAX], nB[KMAX];  // this lane's shell rows after its last proposal
 
+  const bool guided = A.gain != nullptr;
+  if (guided) refresh_gain<KMAX>(A, T, sp, l);
+
   while (acc < A.depth) {
+    if (guided) {
+      // ---- guided: each slot takes the Gumbel-max of -gain/(tau*scale)
+      // over g_cand keyed candidate pairs (same-species candidates skipped)
+      if (rounds > 0 && rounds % A.g_every == 0) refresh_gain<KMAX>(A, T, sp, l);
+      const float* u = A.gain + (size_t)l * N * K;
+      const float inv = (float)(1.0 / (A.g_tau * T.sh->gscale));
+      for (int j = tid; j < task; j += nthr) {
+        float best = -3.0e38f;
+        int2 bp = make_int2(-1, -1);
+        for (int c = 0; c < A.g_cand; ++c) {
+          const uint32_t pi = pos + (uint32_t)(j * A.g_cand + c);
+          uint32_t s1, s2;
+          hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pi, (uint32_t)N, &s1, &s2);
+          const int a = sp[s1], b = sp[s2];
+          if (a != b) {
+            const hea_u32x4 w = hea_stream_block(A.seed, iter, A.phase | 0x100u, (uint32_t)l, pi >> 2);
+            const uint32_t wv = (pi & 3u) == 0 ? w.x : (pi & 3u) == 1 ? w.y : (pi & 3u) == 2 ? w.z : w.w;
+            const float U = ((float)wv + 0.5f) * (1.0f / 2147483648.0f);
+            const float sc = -(u[(size_t)s1 * K + b] + u[(size_t)s2 * K + a]) * inv -
+                             logf(-logf(U));
+            if (sc > best) {
+              best = sc;
+              bp = make_int2((int)s1, (int)s2);
+            }
+          }
+        }
+        T.prop[j] = bp;
+      }
+      pos += (uint32_t)(task * A.g_cand);
+      __syncthreads();
+    }
     // ---- draw `task` valid proposals in stream order (accelerate.cpp:649-655)
-    int found = 0;
+    int found = guided ? task : 0;
     while (found < task) {
       const uint32_t p = pos + tid;
       uint32_t s1, s2;
@@ -356,20 +443,21 @@ __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
       found += tot;
       pos += nthr;
       __syncthreads();
+      if (found >= task) pos = T.sh->next_pos;
     }
-    pos = T.sh->next_pos;
 
     // ---- evaluate; keep the first strict minimum (accelerate.cpp:663-674)
     unsigned long long bkey = ~0ull;
     int bj = 0x7fffffff;
     for (int q = 0; q < nq; ++q) {
       const int j = jl + q * TP;
-      if (j < task) {
+      if (j < task && T.prop[j].x >= 0) {
         const int2 pr = T.prop[j];
-        T.esum[s_me * TP + jl] = eval_shell<KMAX>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
+        T.esum[s_me * TP + jl] =
+            eval_shell<KMAX, EXACT, SELF>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
       }
       __syncthreads();
-      if (s_me == 0 && j < task) {
+      if (s_me == 0 && j < task && T.prop[j].x >= 0) {
         double err = 0.0;
         for (int s = 0; s < S; ++s) err = err + A.weight[s] * T.esum[s * TP + jl];
         const unsigned long long fk = __double_as_longlong(sqrt(err));
@@ -418,10 +506,10 @@ __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
       const int2 pr = T.prop[wj];
       const int a = sp[pr.x], b = sp[pr.y];
       if (jl == wj % TP) {  // the winner's lane of every shell writes its rows
-        if (nq > 1) (void)eval_shell<KMAX>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
+        if (nq > 1) (void)eval_shell<KMAX, EXACT, SELF>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
 #pragma unroll
         for (int x = 0; x < KMAX; ++x) {
-          if (x < K) {
+          if (EXACT || x < K) {
             T.G[T.idx(s_me, a, x)] = nA[x];
             T.G[T.idx(s_me, x, a)] = nA[x];
             T.G[T.idx(s_me, b, x)] = nB[x];
@@ -465,27 +553,94 @@ __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
   }
 }
 
-template <int KMAX, bool SP_LDS>
-const void* mc_fn() {
-  return reinterpret_cast<const void*>(k_local_mc<KMAX, SP_LDS>);
+template <int KMAX, bool SP_LDS, bool EXACT, bool SELF>
+__global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  mc_lattice<KMAX, SP_LDS, EXACT, SELF>(A, blockIdx.x, smem);
 }
 
-const void* mc_fn_for(int K, bool sp_lds) {
-  switch (hea_mc_kmax(K)) {
-    case 2: return sp_lds ? mc_fn<2, true>() : mc_fn<2, false>();
-    case 3: return sp_lds ? mc_fn<3, true>() : mc_fn<3, false>();
-    case 4: return sp_lds ? mc_fn<4, true>() : mc_fn<4, false>();
-    case 5: return sp_lds ? mc_fn<5, true>() : mc_fn<5, false>();
-    case 6: return sp_lds ? mc_fn<6, true>() : mc_fn<6, false>();
-    case 8: return sp_lds ? mc_fn<8, true>() : mc_fn<8, false>();
-    default: return nullptr;
+// Persistent local search over many problems: every workgroup takes
+// (problem, lattice) items from a device queue, longest first, until the
+// queue is empty.  Per-problem fields come from B.probs; run fields from
+// B.run.  Problem p draws from seed B.run.seed + p.
+template <int KMAX, bool SP_LDS>
+__global__ __launch_bounds__(1024) void k_local_mc_batch(HeaMcBatchArgs B) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int item_sh;
+  for (;;) {
+    if (threadIdx.x == 0) item_sh = (int)atomicAdd(B.queue, 1u);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(item_sh);
+    __syncthreads();
+    if (item >= B.n_items) break;
+    const int2 it = B.items[item];
+    HeaMcArgs A = B.probs[it.x];
+    A.task = B.run.task;
+    A.depth = B.run.depth;
+    A.tp = B.run.tp;
+    A.phase = B.run.phase;
+    A.thr = B.run.thr;
+    A.iter_ptr = B.run.iter_ptr;
+    A.seed = B.run.seed + (uint64_t)it.x;
+    A.g_every = B.run.g_every;
+    A.g_cand = B.run.g_cand;
+    A.g_tau = B.run.g_tau;
+    mc_lattice<KMAX, SP_LDS, false, false>(A, it.y, smem);
+    __syncthreads();
   }
+}
+
+template <int KMAX, bool SP_LDS, bool EXACT, bool SELF>
+const void* mc_fn() {
+  return reinterpret_cast<const void*>(k_local_mc<KMAX, SP_LDS, EXACT, SELF>);
+}
+
+// Kernel variant: exact-K instantiations for the common case, a guarded
+// K <= KMAX one for K = 7 and mixed batches, and a self-neighbour one for
+// cells so small that a site lists itself.
+const void* mc_fn_for(int K, bool sp_lds, bool exact_ok, bool self) {
+  if (self) return sp_lds ? mc_fn<8, true, false, true>() : mc_fn<8, false, false, true>();
+  if (exact_ok) {
+    switch (K) {
+      case 2: return sp_lds ? mc_fn<2, true, true, false>() : mc_fn<2, false, true, false>();
+      case 3: return sp_lds ? mc_fn<3, true, true, false>() : mc_fn<3, false, true, false>();
+      case 4: return sp_lds ? mc_fn<4, true, true, false>() : mc_fn<4, false, true, false>();
+      case 5: return sp_lds ? mc_fn<5, true, true, false>() : mc_fn<5, false, true, false>();
+      case 6: return sp_lds ? mc_fn<6, true, true, false>() : mc_fn<6, false, true, false>();
+      case 8: return sp_lds ? mc_fn<8, true, true, false>() : mc_fn<8, false, true, false>();
+      default: break;
+    }
+  }
+  if (K <= 6) return sp_lds ? mc_fn<6, true, false, false>() : mc_fn<6, false, false, false>();
+  if (K <= 8) return sp_lds ? mc_fn<8, true, false, false>() : mc_fn<8, false, false, false>();
+  return nullptr;
+}
+
+const void* mc_batch_fn_for(int kmax, bool sp_lds) {
+  if (kmax <= 6)
+    return sp_lds ? reinterpret_cast<const void*>(k_local_mc_batch<6, true>)
+                  : reinterpret_cast<const void*>(k_local_mc_batch<6, false>);
+  return sp_lds ? reinterpret_cast<const void*>(k_local_mc_batch<8, true>)
+                : reinterpret_cast<const void*>(k_local_mc_batch<8, false>);
 }
 
 }  // namespace
 
-int hea_mc_kmax(int K) {
-  return K <= 2 ? 2 : K <= 3 ? 3 : K <= 4 ? 4 : K <= 5 ? 5 : K <= 6 ? 6 : K <= 8 ? 8 : 0;
+hipError_t hea_prepare_local_mc_batch(int kmax, bool sp_lds, size_t lds) {
+  return hipFuncSetAttribute(mc_batch_fn_for(kmax, sp_lds),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_lds, int grid,
+                                     int threads, size_t lds, hipStream_t st) {
+  void* args[] = {const_cast<HeaMcBatchArgs*>(&b)};
+  return hipLaunchKernel(mc_batch_fn_for(kmax, sp_lds), dim3(grid), dim3(threads), args, lds, st);
+}
+
+// Table width of the kernel chosen for K species (see mc_fn_for).
+int hea_mc_kmax(int K, bool self) {
+  if (self || K == 7) return 8;
+  return K <= 6 ? K : 8;
 }
 
 // Proposals evaluated per pass (lanes per shell): a multiple of 64, at most
@@ -509,15 +664,15 @@ size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds) {
 
 // Raises the kernel's dynamic-LDS limit; call before launching (and before
 // any graph capture that contains the launch).
-hipError_t hea_prepare_local_mc(int K, bool sp_lds, size_t lds) {
-  const void* fn = mc_fn_for(K, sp_lds);
+hipError_t hea_prepare_local_mc(int K, bool sp_lds, bool self, size_t lds) {
+  const void* fn = mc_fn_for(K, sp_lds, true, self);
   if (!fn) return hipErrorInvalidValue;
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, size_t lds,
+hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self, size_t lds,
                                hipStream_t st) {
-  const void* fn = mc_fn_for(K, sp_lds);
+  const void* fn = mc_fn_for(K, sp_lds, true, self);
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<HeaMcArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(a.L), dim3(a.S * a.tp), args, lds, st);

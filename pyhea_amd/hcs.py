@@ -133,6 +133,10 @@ class Session:
                                            converge_depth, thr, seed, C.byref(h)))
         self._h = h
 
+    def guide(self, every: int = 16, candidates: int = 16, tau: float = 0.1) -> None:
+        """Switch both local searches to gradient-guided proposals."""
+        N.check(N.lib().hea_session_guide(self._h, every, candidates, tau))
+
     def init(self) -> None:
         N.check(N.lib().hea_session_init(self._h))
 
@@ -166,6 +170,57 @@ class Session:
     def close(self) -> None:
         if getattr(self, "_h", None):
             N.lib().hea_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many problems searched together (hea_batch_*): one persistent
+    local-search launch per phase over every (problem, lattice) item.
+    Problem p draws from seed ``seed + p``."""
+
+    def __init__(self, problems, solutions: int, parallel_task: int, converge_depth: int,
+                 thr: float = 1e-3, seed: int = 0x5EED):
+        self.problems = list(problems)
+        self.solutions = int(solutions)
+        arr = (C.c_void_p * len(self.problems))(*[p.handle for p in self.problems])
+        h = C.c_void_p()
+        N.check(N.lib().hea_batch_create(arr, len(self.problems), solutions, parallel_task,
+                                         converge_depth, thr, seed, C.byref(h)))
+        self._h = h
+
+    def guide(self, every: int = 16, candidates: int = 16, tau: float = 0.1) -> None:
+        N.check(N.lib().hea_batch_guide(self._h, every, candidates, tau))
+
+    def set_grid(self, grid: int) -> None:
+        N.check(N.lib().hea_batch_grid(self._h, int(grid)))
+
+    def init(self) -> None:
+        N.check(N.lib().hea_batch_init(self._h))
+
+    def step(self, n_iter: int = 1) -> None:
+        N.check(N.lib().hea_batch_step(self._h, n_iter))
+
+    def sync(self) -> None:
+        N.check(N.lib().hea_batch_sync(self._h))
+
+    def read(self, p: int):
+        prob = self.problems[p]
+        lat = np.empty((self.solutions, prob.n_sites), dtype=np.uint8)
+        fit = np.empty(self.solutions, dtype=np.float64)
+        st = N.HeaStats()
+        N.check(N.lib().hea_batch_read(self._h, p, _ptr(lat, _u8p), _ptr(fit, _dp),
+                                       C.byref(st)))
+        return lat, fit, st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.lib().hea_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -1,0 +1,50 @@
+"""Batched search: every problem of a batch equals the plain search of that
+problem with seed + p, bit for bit, whatever the persistent grid size."""
+import numpy as np
+import pytest
+
+from pyhea_amd.hcs import Batch, Problem
+from pyhea_amd.problem import make_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    rng = np.random.default_rng(4)
+    out = []
+    for i in range(6):
+        typ = ["FCC", "BCC"][i % 2]
+        K = int(rng.integers(2, 7))
+        out.append(make_problem(typ, int(rng.integers(2, 5)), rng.dirichlet([2.0] * K) + 0.05,
+                                shells="true", target_seed=50 + i))
+    return out
+
+
+def _bits(a):
+    return np.asarray(a, np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("grid", [None, 1, 3])
+def test_batch_equals_plain(gpu, grid):
+    specs = _specs()
+    probs = [Problem.from_spec(s) for s in specs]
+    L, I, T, D, seed = 5, 2, 32, 2, 900
+    b = Batch(probs, L, T, D, seed=seed)
+    if grid:
+        b.set_grid(grid)
+    b.init()
+    b.step(I)
+    for q, p in enumerate(probs):
+        lb, fb, sb = b.read(q)
+        lp, fp, sp = p.run_hcs(L, I, T, D, seed=seed + q)
+        assert np.array_equal(lb, lp), q
+        assert np.array_equal(_bits(fb), _bits(fp)), q
+        assert sb["evals"] == sp["evals"] and sb["accepts"] == sp["accepts"], q
+    b.close()
+
+
+def test_batch_rejects_mixed_shell_counts(gpu):
+    a = make_problem("FCC", 2, [0.5, 0.5], weight=[1.0, 1.0], shells="true", target_seed=1)
+    c = make_problem("FCC", 2, [0.5, 0.5], weight=[1.0], shells="true", target_seed=1)
+    with pytest.raises(ValueError):
+        Batch([Problem.from_spec(a), Problem.from_spec(c)], 2, 8, 1)

@@ -1,0 +1,112 @@
+"""Differentiable relaxation: HIP forward and hand-written HIP backward
+against a plain PyTorch FP64 reference, against FP64 central finite
+differences (h = 1e-6), and the one-hot limit against integer pair counts."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import Oracle
+from pyhea_amd.hcs import Problem
+from pyhea_amd.problem import make_problem
+from pyhea_amd.relax import RelaxObjective, SoftPairCounts, site_gain
+
+pytestmark = pytest.mark.gpu
+
+
+def _spec(typ="FCC", cell=3, K=4, S=3, seed=1):
+    fr = np.random.default_rng(seed).dirichlet([3.0] * K) + 0.05
+    return make_problem(typ, cell, fr, weight=[4.0, 1.0, 0.5][:S], shells="true",
+                        target_seed=seed)
+
+
+def torch_reference_counts(logits64, nbr, shell_sizes):
+    """n[s] = P^T Q_s with Q_s[i] = sum_{j in N_s(i)} p_j, all in FP64."""
+    p = torch.softmax(logits64, dim=1)
+    out, off = [], 0
+    for z in shell_sizes:
+        idx = torch.as_tensor(nbr[:, off:off + z], dtype=torch.long, device=p.device)
+        q = p[idx].sum(1)
+        out.append(p.T @ q)
+        off += z
+    return torch.stack(out)
+
+
+@pytest.mark.parametrize("typ,cell,K,S", [("FCC", 3, 4, 3), ("BCC", 4, 5, 2), ("FCC", 2, 6, 1)])
+def test_forward_matches_torch(gpu, typ, cell, K, S):
+    spec = _spec(typ, cell, K, S)
+    prob = Problem.from_spec(spec)
+    th = torch.randn(spec.n_sites, K, dtype=torch.float64, device="cuda")
+    n64 = SoftPairCounts.apply(th, prob)
+    ref = torch_reference_counts(th, spec.nbr, spec.shell_sizes)
+    assert torch.allclose(n64, ref, rtol=1e-12, atol=1e-12)
+    n32 = SoftPairCounts.apply(th.float(), prob)
+    assert torch.allclose(n32, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_one_hot_limit_is_integer_pair_counts(gpu):
+    spec = _spec("BCC", 4, 3, 3)
+    prob, o = Problem.from_spec(spec), Oracle.from_spec(spec)
+    lat = o.permute(3, 0, 0, 0)
+    th = torch.full((spec.n_sites, 3), -60.0, dtype=torch.float64, device="cuda")
+    th[torch.arange(spec.n_sites), torch.as_tensor(lat, dtype=torch.long)] = 60.0
+    n = SoftPairCounts.apply(th, prob).cpu().numpy()
+    assert np.allclose(n, o.pair_counts(lat), rtol=0, atol=1e-9)
+    # and the relaxed objective equals the reference fitness squared
+    f2 = RelaxObjective(prob)(th).item()
+    assert abs(f2 - o.fitness(lat)[0] ** 2) < 1e-9 * max(1.0, f2)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 2e-4)])
+def test_backward_matches_torch_autograd(gpu, dtype, tol):
+    spec = _spec("FCC", 3, 5, 3, seed=2)
+    prob = Problem.from_spec(spec)
+    obj = RelaxObjective(prob)
+    th0 = torch.randn(spec.n_sites, 5, dtype=torch.float64, device="cuda")
+    th = th0.to(dtype).requires_grad_(True)
+    obj(th).backward()
+    ref_th = th0.clone().requires_grad_(True)
+    obj.from_counts(torch_reference_counts(ref_th, spec.nbr, spec.shell_sizes)).backward()
+    g, r = th.grad.double(), ref_th.grad
+    assert (g - r).abs().max().item() <= tol * r.abs().max().item()
+
+
+def test_backward_matches_central_differences(gpu):
+    spec = _spec("BCC", 3, 4, 3, seed=3)
+    prob = Problem.from_spec(spec)
+    obj = RelaxObjective(prob)
+    th = torch.randn(spec.n_sites, 4, dtype=torch.float64, device="cuda").requires_grad_(True)
+    obj(th).backward()
+    grad = th.grad.clone()
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for _ in range(24):
+        i, c = int(rng.integers(spec.n_sites)), int(rng.integers(4))
+        tp, tm = th.detach().clone(), th.detach().clone()
+        tp[i, c] += h
+        tm[i, c] -= h
+        fd = (obj(tp).item() - obj(tm).item()) / (2 * h)
+        assert abs(fd - grad[i, c].item()) <= 1e-6 * max(1.0, abs(fd)) + 1e-8, (i, c)
+
+
+def test_site_gain_is_one_hot_gradient(gpu):
+    """u[k][b] = dF2/dp[k][b] - dF2/dp[k][a_k] at the one-hot lattice."""
+    spec = _spec("FCC", 3, 4, 3, seed=4)
+    prob, o = Problem.from_spec(spec), Oracle.from_spec(spec)
+    obj = RelaxObjective(prob)
+    lats = np.stack([o.permute(8, 0, 0, l) for l in range(3)])
+    dn = torch.stack([obj.count_gradient(torch.as_tensor(o.pair_counts(l), dtype=torch.float64,
+                                                          device="cuda")) for l in lats])
+    u = site_gain(prob, torch.as_tensor(lats, device="cuda"), dn).double()
+    for l in range(3):
+        p = torch.nn.functional.one_hot(torch.as_tensor(lats[l], dtype=torch.long), 4)
+        p = p.double().cuda().requires_grad_(True)
+        out, off = [], 0
+        for z in spec.shell_sizes:
+            idx = torch.as_tensor(spec.nbr[:, off:off + z], dtype=torch.long, device="cuda")
+            out.append(p.T @ p[idx].sum(1))
+            off += z
+        obj.from_counts(torch.stack(out)).backward()
+        g = p.grad
+        want = g - g.gather(1, torch.as_tensor(lats[l], dtype=torch.long,
+                                               device="cuda")[:, None])
+        assert torch.allclose(u[l], want, rtol=1e-5, atol=1e-6 * want.abs().max().item())
