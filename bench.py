@@ -43,12 +43,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=49)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain-bcc16", choices=["plain-bcc16", "plain-fcc10"])
+    ap.add_argument("--workload", default="plain-bcc16",
+                    choices=["plain-bcc16", "plain-fcc10", "ttf-fcc30", "ttf-fcc50", "batched-64"])
+    ap.add_argument("--budget-s", type=float, default=60.0,
+                    help="time-to-fitness workloads: wall-clock budget per mode")
+    ap.add_argument("--thr", type=float, default=1e-3)
+    ap.add_argument("--dump-problem", default=None,
+                    help="write the workload's problem for pyhea_amd/hea_profile and exit")
     ap.add_argument("--profile-steps", type=int, default=3,
                     help="eager iterations timed per launch for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="approximate CPU work of the baseline sample (0 disables it)")
     ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape autotuner (default tile)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a copy "
                          "kernel of known size for calibrating HBM byte counters)")
@@ -118,11 +126,69 @@ def cpu_worker(path: str, kind: str, T: int, D: int, seed: int, wid: int) -> Non
         o.local_mc(d["lat"], d["fit"], T, D, seed, 0, 2, sort=False)
 
 
+def time_to_fitness(a):
+    """Plain vs guided search on one problem: global iterations until the
+    population best reaches --thr or --budget-s runs out; the device clock
+    stamps the first local-search accept at or below the threshold."""
+    from pyhea_amd.hcs import Problem
+    from pyhea_amd.problem import workload
+
+    wid = {"ttf-fcc30": 3, "ttf-fcc50": 4}[a.workload]
+    spec, run = workload(wid)
+    L, T, D, seed = run["solutions"], run["parallel_task"], run["converge_depth"], run["seed"]
+    prob = Problem.from_spec(spec)
+    modes = {}
+    for mode in ("plain", "guided"):
+        s = prob.session(L, T, D, thr=a.thr, seed=seed)
+        if mode == "guided":
+            s.guide(every=16, candidates=16, tau=0.1)
+        t0 = time.perf_counter()
+        s.init()
+        trace = []
+        while True:
+            s.step(1)
+            best = s.best()
+            trace.append((round(time.perf_counter() - t0, 3), best))
+            if best <= a.thr or time.perf_counter() - t0 > a.budget_s:
+                break
+        _, fit, st = s.read()
+        modes[mode] = {"best": float(fit[0]), "iterations": len(trace), "wall_s": trace[-1][0],
+                       "evals": st["evals"], "evals_per_s": st["evals"] / trace[-1][0],
+                       "time_to_thr_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,
+                       "trace": trace}
+        s.close()
+    # time each mode needed to reach the plain run's final best
+    target = modes["plain"]["best"]
+    for m in modes.values():
+        m["time_to_plain_best_s"] = next((t for t, f in m["trace"] if f <= target), None)
+    tp, tg = modes["plain"]["time_to_plain_best_s"], modes["guided"]["time_to_plain_best_s"]
+    out = {"metric": "time to fitness (guided vs plain)", "workload": a.workload,
+           "n_sites": spec.n_sites, "solutions": L, "parallel_task": T, "thr": a.thr,
+           "speedup_to_plain_best": (tp / tg) if (tp and tg) else None, "modes": modes}
+    print(json.dumps(out))
+
+
 def main():
     if len(sys.argv) == 8 and sys.argv[1] == "--cpu-worker":
         cpu_worker(sys.argv[2], sys.argv[3], *map(int, sys.argv[4:8]))
         return
     a = parse()
+    if a.dump_problem:
+        from pyhea_amd.problem import workload
+
+        spec, _ = workload({"plain-bcc16": 1, "plain-fcc10": 0, "ttf-fcc30": 3,
+                            "ttf-fcc50": 4}[a.workload])
+        with open(a.dump_problem, "wb") as f:
+            z = list(spec.shell_sizes) + [0] * (3 - spec.S)
+            f.write(np.array([spec.n_sites, spec.K, spec.S] + z, np.int32).tobytes())
+            f.write(np.ascontiguousarray(spec.nbr, np.int32).tobytes())
+            f.write(np.array(spec.counts, np.int32).tobytes())
+            f.write(np.array(spec.weight, np.float64).tobytes())
+            f.write(np.ascontiguousarray(spec.target, np.float64).tobytes())
+        return
+    if a.workload.startswith("ttf-"):
+        time_to_fitness(a)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the population would shard one sub-population per GPU)"}))
@@ -138,7 +204,14 @@ def main():
     spec, run = workload(wid)
     L, T, D, seed = run["solutions"], run["parallel_task"], run["converge_depth"], run["seed"]
     prob = Problem.from_spec(spec)
+    tile = None
+    if not a.no_tune:  # launch shape only; results are identical for every tile
+        from pyhea_amd.autotune import tune_session
+
+        tile = tune_session(prob, L, T, D, seed=seed, cache_path=None)["tile"]
     sess = prob.session(L, T, D, thr=1e-3, seed=seed)
+    if tile:
+        sess.tile(tile)
     t_start = time.perf_counter()
     sess.init()
     sess.step(a.warmup)
@@ -205,6 +278,7 @@ def main():
             "converge_depth": D,
             "total_iter": a.warmup + a.steps,
             "seed": seed,
+            "tile": tile,
         },
         "best_fitness": float(fit[0]),
         "time_to_fitness_1e-3_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,

@@ -132,6 +132,11 @@ int hea_session_create(hea_problem p, int32_t solutions, int32_t parallel_task,
                        int32_t converge_depth, double thr, uint64_t seed, hea_session* out);
 /* Initial population (accelerate.cpp:737); starts the session clock. */
 int hea_session_init(hea_session s);
+/* Launch shape of the session's local searches, no counterpart in the
+ * reference: `tile` proposals are evaluated per pass (64, 128, 192 or 256;
+ * shells * tile <= 1024).  Results do not depend on it; the autotuner
+ * (pyhea_amd/autotune.py) picks it. */
+int hea_session_tile(hea_session s, int32_t tile);
 /* Guided proposals, no counterpart in the reference: every `every` rounds a
  * local search recomputes site gains u[k][c] (first-order change of the
  * fitness squared when site k turns into species c, from the relaxation's
@@ -170,6 +175,8 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t solution
 /* Sets the number of persistent workgroups (launch shape chosen by the
  * autotuner; results do not depend on it). */
 int hea_batch_grid(hea_batch b, int32_t grid);
+/* Proposals per pass of the batch's persistent launch (see hea_session_tile). */
+int hea_batch_tile(hea_batch b, int32_t tile);
 /* Guided proposals for every problem of the batch (see hea_session_guide). */
 int hea_batch_guide(hea_batch b, int32_t every, int32_t candidates, double tau);
 int hea_batch_init(hea_batch b);

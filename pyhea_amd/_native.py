@@ -31,6 +31,7 @@ EXPORTS = (
     "hea_generate",
     "hea_session_create",
     "hea_session_init",
+    "hea_session_tile",
     "hea_session_guide",
     "hea_session_step",
     "hea_session_step_timed",
@@ -40,6 +41,7 @@ EXPORTS = (
     "hea_session_destroy",
     "hea_batch_create",
     "hea_batch_grid",
+    "hea_batch_tile",
     "hea_batch_guide",
     "hea_batch_init",
     "hea_batch_step",
@@ -107,6 +109,7 @@ def _declare(lib: C.CDLL) -> None:
         "hea_generate": (C.c_int, [P, I32, U64, U32, U32, u8p, dp]),
         "hea_session_create": (C.c_int, [P, I32, I32, I32, D, U64, C.POINTER(P)]),
         "hea_session_init": (C.c_int, [P]),
+        "hea_session_tile": (C.c_int, [P, I32]),
         "hea_session_guide": (C.c_int, [P, I32, I32, D]),
         "hea_session_step": (C.c_int, [P, I32]),
         "hea_session_step_timed": (C.c_int, [P, I32, C.POINTER(C.c_float),
@@ -117,6 +120,7 @@ def _declare(lib: C.CDLL) -> None:
         "hea_session_destroy": (C.c_int, [P]),
         "hea_batch_create": (C.c_int, [C.POINTER(P), I32, I32, I32, I32, D, U64, C.POINTER(P)]),
         "hea_batch_grid": (C.c_int, [P, I32]),
+        "hea_batch_tile": (C.c_int, [P, I32]),
         "hea_batch_guide": (C.c_int, [P, I32, I32, D]),
         "hea_batch_init": (C.c_int, [P]),
         "hea_batch_step": (C.c_int, [P, I32]),

@@ -103,6 +103,17 @@ def build_bind(verbose: bool = False) -> str:
     return out
 
 
+def build_profile_driver(verbose: bool = False) -> str:
+    """pyhea_amd/hea_profile: a Python-free driver for hardware-counter runs."""
+    out = os.path.join(PKG, "hea_profile")
+    _run([_hipcc(), "-O2", "-std=c++17", os.path.join(CSRC, "hea_profile_main.cpp"),
+          f"-L{PKG}", "-lpyhea_hcs", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("[build] hea_profile", flush=True)
+    return out
+
+
 def build_oracle(with_reference: bool | None = None) -> None:
     """Test infrastructure: the CPU oracle and, where the reference tree
     exists, the reference harness (outputs under oracle/)."""
@@ -118,6 +129,7 @@ def build_oracle(with_reference: bool | None = None) -> None:
 def build_all(verbose: bool = False) -> None:
     build_hip(verbose)
     build_bind(verbose)
+    build_profile_driver(verbose)
 
 
 if __name__ == "__main__":

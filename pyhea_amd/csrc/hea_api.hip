@@ -223,6 +223,7 @@ struct hea_session_s {
   int iters_done = 0;
   DevBuf<float> gain;  // guided mode
   GuideCfg guide;
+  int tp = 0;          // proposals per pass (0: default)
 };
 
 namespace {
@@ -352,7 +353,7 @@ void fill_problem_args(hea_problem p, HeaMcArgs& a) {
 int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, int64_t* stats,
               const uint32_t* iter_ptr, unsigned long long* first_hit, int L, int task,
               int depth, uint64_t seed, uint32_t phase, double thr, size_t lds, bool sp_lds,
-              hipStream_t st, const GuideCfg& gd = GuideCfg()) {
+              hipStream_t st, const GuideCfg& gd = GuideCfg(), int tp = 0) {
   HeaMcArgs a;
   a.pc = p->d_const.p;
   a.nbr = p->d_nbr.p;
@@ -366,7 +367,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.L = L;
   a.task = task;
   a.depth = depth;
-  a.tp = hea_mc_tp(task, p->hc.S);
+  a.tp = tp > 0 ? tp : hea_mc_tp(task, p->hc.S);
   a.phase = phase;
   a.thr = thr;
   a.gain = gd.gain;
@@ -398,10 +399,10 @@ int check_run_args(hea_problem p, int L, int task, int depth) {
 }
 
 // LDS bytes of a local-search launch; species live in LDS when they fit.
-int mc_lds(hea_problem p, int task, size_t* lds, bool* sp_lds) {
+int mc_lds(hea_problem p, int task, size_t* lds, bool* sp_lds, int tp = 0) {
   const bool self = p->hc.self_nbr != 0;
   const int km = hea_mc_kmax(p->hc.K, self);
-  const int tp = hea_mc_tp(task, p->hc.S);
+  if (tp <= 0) tp = hea_mc_tp(task, p->hc.S);
   *sp_lds = !p->force_hbm_species;
   *lds = hea_mc_lds_bytes(km, task, tp, p->hc.N, *sp_lds);
   if ((int64_t)*lds > p->max_lds) {
@@ -692,7 +693,7 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
   }
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
                       s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, s->sp_lds, st,
-                      s->guide)))
+                      s->guide, s->tp)))
     return rc;
   if (hk) {
     HEA_HIP(hipEventRecord(hk->ev[1], st));
@@ -710,7 +711,7 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
   if (hk) HEA_HIP(hipEventRecord(hk->ev[2], st));
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
                       s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, s->sp_lds, st,
-                      s->guide)))
+                      s->guide, s->tp)))
     return rc;
   if (hk) {
     HEA_HIP(hipEventRecord(hk->ev[3], st));
@@ -766,6 +767,28 @@ int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, do
     return fail(HEA_EHIP, hipGetErrorString(e));
   }
   *out = s;
+  return HEA_OK;
+}
+
+static int valid_tp(int tp, int S) { return tp >= 64 && tp <= 256 && tp % 64 == 0 && S * tp <= 1024; }
+
+int hea_session_tile(hea_session s, int32_t tp) {
+  if (!s || !valid_tp(tp, s->p->hc.S))
+    return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256 with shells*tile <= 1024");
+  HEA_HIP(hipSetDevice(s->p->device));
+  int rc;
+  size_t lds;
+  bool sp;
+  if ((rc = mc_lds(s->p, s->task, &lds, &sp, tp))) return rc;
+  s->tp = tp;
+  s->lds = lds;
+  s->sp_lds = sp;
+  if (s->exec) {
+    (void)hipGraphExecDestroy(s->exec);
+    (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+  }
   return HEA_OK;
 }
 
@@ -1126,6 +1149,30 @@ int hea_batch_grid(hea_batch b, int32_t grid) {
   if (!b || grid < 1) return fail(HEA_EINVAL, "bad arguments");
   b->grid = std::min(grid, (int32_t)(b->probs.size() * b->L));
   if (b->exec) {  // the graph bakes the launch shape in: capture again
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  return HEA_OK;
+}
+
+int hea_batch_tile(hea_batch b, int32_t tp) {
+  if (!b || !valid_tp(tp, b->S))
+    return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256 with shells*tile <= 1024");
+  int maxN = 0;
+  for (auto q : b->probs) maxN = std::max(maxN, q->hc.N);
+  b->tp = tp;
+  b->threads = b->S * tp;
+  b->sp_lds = !b->probs[0]->force_hbm_species;
+  b->lds = hea_mc_lds_bytes(b->kmax, b->task, tp, maxN, b->sp_lds);
+  if ((int64_t)b->lds > b->probs[0]->max_lds) {
+    b->sp_lds = false;
+    b->lds = hea_mc_lds_bytes(b->kmax, b->task, tp, maxN, false);
+  }
+  HEA_HIP(hipSetDevice(b->probs[0]->device));
+  HEA_HIP(hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds));
+  if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;

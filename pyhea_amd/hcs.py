@@ -133,6 +133,10 @@ class Session:
                                            converge_depth, thr, seed, C.byref(h)))
         self._h = h
 
+    def tile(self, tp: int) -> None:
+        """Proposals evaluated per pass (launch shape only)."""
+        N.check(N.lib().hea_session_tile(self._h, int(tp)))
+
     def guide(self, every: int = 16, candidates: int = 16, tau: float = 0.1) -> None:
         """Switch both local searches to gradient-guided proposals."""
         N.check(N.lib().hea_session_guide(self._h, every, candidates, tau))
@@ -196,6 +200,9 @@ class Batch:
 
     def guide(self, every: int = 16, candidates: int = 16, tau: float = 0.1) -> None:
         N.check(N.lib().hea_batch_guide(self._h, every, candidates, tau))
+
+    def tile(self, tp: int) -> None:
+        N.check(N.lib().hea_batch_tile(self._h, int(tp)))
 
     def set_grid(self, grid: int) -> None:
         N.check(N.lib().hea_batch_grid(self._h, int(grid)))

@@ -48,3 +48,37 @@ def test_batch_rejects_mixed_shell_counts(gpu):
     c = make_problem("FCC", 2, [0.5, 0.5], weight=[1.0], shells="true", target_seed=1)
     with pytest.raises(ValueError):
         Batch([Problem.from_spec(a), Problem.from_spec(c)], 2, 8, 1)
+
+
+@pytest.mark.parametrize("tile", [64, 128, 256])
+def test_launch_shape_does_not_change_results(gpu, tile):
+    specs = _specs()[:3]
+    probs = [Problem.from_spec(s) for s in specs]
+    L, I, T, D, seed = 4, 2, 200, 2, 31
+    b = Batch(probs, L, T, D, seed=seed)
+    b.tile(tile)
+    b.set_grid(7)
+    b.init()
+    b.step(I)
+    for q, p in enumerate(probs):
+        s = p.session(L, T, D, seed=seed + q)
+        s.tile(tile)
+        s.init()
+        s.step(I)
+        ls, fs, _ = s.read()
+        lp, fp, _ = p.run_hcs(L, I, T, D, seed=seed + q)  # default tile
+        lb, fb, _ = b.read(q)
+        assert np.array_equal(ls, lp) and np.array_equal(_bits(fs), _bits(fp))
+        assert np.array_equal(lb, lp) and np.array_equal(_bits(fb), _bits(fp))
+    b.close()
+
+
+def test_autotuner_picks_a_valid_shape(gpu, tmp_path):
+    from pyhea_amd.autotune import tune_batch
+
+    probs = [Problem.from_spec(s) for s in _specs()[:2]]
+    best = tune_batch(probs, 4, 64, 2, cache_path=str(tmp_path / "t.json"), grids=[4, 8],
+                      tiles=[64, 128])
+    assert best["tile"] in (64, 128) and best["grid"] in (4, 8)
+    again = tune_batch(probs, 4, 64, 2, cache_path=str(tmp_path / "t.json"))
+    assert again == best
