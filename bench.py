@@ -141,7 +141,7 @@ def time_to_fitness(a):
     for mode in ("plain", "guided"):
         s = prob.session(L, T, D, thr=a.thr, seed=seed)
         if mode == "guided":
-            s.guide(every=16, candidates=16, tau=0.1)
+            s.guide(candidates=8, tau=0.1)
         t0 = time.perf_counter()
         s.init()
         trace = []

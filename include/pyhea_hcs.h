@@ -137,14 +137,16 @@ int hea_session_init(hea_session s);
  * shells * tile <= 1024).  Results do not depend on it; the autotuner
  * (pyhea_amd/autotune.py) picks it. */
 int hea_session_tile(hea_session s, int32_t tile);
-/* Guided proposals, no counterpart in the reference: every `every` rounds a
- * local search recomputes site gains u[k][c] (first-order change of the
- * fitness squared when site k turns into species c, from the relaxation's
- * gradient at the current lattice); each of the parallel_task proposals of a
- * round is the Gumbel-max of -(u[i][b] + u[j][a]) / (tau * scale) over
- * `candidates` keyed random pairs.  Acceptance stays the exact rule of
- * accelerate.cpp:663-689.  Call before hea_session_init/step. */
-int hea_session_guide(hea_session s, int32_t every, int32_t candidates, double tau);
+/* Guided proposals, no counterpart in the reference.  The local search keeps
+ * every site's neighbour species counts and the gradient of the fitness
+ * squared with respect to the pair counts (the relaxation's gradient at the
+ * current lattice) up to date; the predicted first-order change of swapping
+ * i (species a) with j (species b) is u[i][b] + u[j][a].  Each of the
+ * parallel_task proposals of a round is the Gumbel-max of
+ * -(u[i][b] + u[j][a]) / (tau * scale) over `candidates` keyed random pairs.
+ * Acceptance stays the exact rule of accelerate.cpp:663-689.  Needs K <= 6
+ * and no self neighbours.  Call before hea_session_init/step. */
+int hea_session_guide(hea_session s, int32_t candidates, double tau);
 /* Runs `n_iter` global iterations asynchronously on the session stream. */
 int hea_session_step(hea_session s, int32_t n_iter);
 /* Measurement helper with no counterpart in the reference: runs n_iter
@@ -178,7 +180,7 @@ int hea_batch_grid(hea_batch b, int32_t grid);
 /* Proposals per pass of the batch's persistent launch (see hea_session_tile). */
 int hea_batch_tile(hea_batch b, int32_t tile);
 /* Guided proposals for every problem of the batch (see hea_session_guide). */
-int hea_batch_guide(hea_batch b, int32_t every, int32_t candidates, double tau);
+int hea_batch_guide(hea_batch b, int32_t candidates, double tau);
 int hea_batch_init(hea_batch b);
 int hea_batch_step(hea_batch b, int32_t n_iter);
 int hea_batch_sync(hea_batch b);

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -181,8 +182,8 @@ struct DevBuf {
 
 // Guided-proposal settings of a local-search launch (gain == nullptr: plain).
 struct GuideCfg {
-  float* gain = nullptr;
-  int every = 0, cand = 0;
+  uint32_t* hist = nullptr;  // [L][N][S] packed neighbour species counts
+  int cand = 0;
   double tau = 0.0;
 };
 
@@ -221,9 +222,9 @@ struct hea_session_s {
   DevBuf<uint32_t> iter;
   DevBuf<unsigned long long> clk;  // [0] start, [1] first fitness <= thr
   int iters_done = 0;
-  DevBuf<float> gain;  // guided mode
+  DevBuf<uint32_t> hist;  // guided mode
   GuideCfg guide;
-  int tp = 0;          // proposals per pass (0: default)
+  int tp = 0;             // proposals per pass (0: default)
 };
 
 namespace {
@@ -370,8 +371,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.tp = tp > 0 ? tp : hea_mc_tp(task, p->hc.S);
   a.phase = phase;
   a.thr = thr;
-  a.gain = gd.gain;
-  a.g_every = gd.every;
+  a.hist = gd.hist;
   a.g_cand = gd.cand;
   a.g_tau = gd.tau;
   const HeaConst& c = p->hc;
@@ -792,14 +792,26 @@ int hea_session_tile(hea_session s, int32_t tp) {
   return HEA_OK;
 }
 
-int hea_session_guide(hea_session s, int32_t every, int32_t candidates, double tau) {
-  if (!s || every < 1 || candidates < 1 || !(tau > 0.0))
-    return fail(HEA_EINVAL, "guide needs every >= 1, candidates >= 1, tau > 0");
+static int check_guide(hea_problem p, int candidates, double tau) {
+  if (candidates < 1 || !(tau > 0.0)) return fail(HEA_EINVAL, "guide needs candidates >= 1, tau > 0");
+  if (p->hc.K > 6) return fail(HEA_EUNSUPPORTED, "guided proposals support up to 6 species");
+  if (p->hc.self_nbr) return fail(HEA_EUNSUPPORTED, "guided proposals need cells without self neighbours");
+  return HEA_OK;
+}
+
+int hea_session_guide(hea_session s, int32_t candidates, double tau) {
+  if (!s) return fail(HEA_EINVAL, "null session");
+  int rc = check_guide(s->p, candidates, tau);
+  if (rc) return rc;
   HEA_HIP(hipSetDevice(s->p->device));
-  const size_t n = (size_t)s->L * s->p->hc.N * s->p->hc.K;
-  if (s->gain.n < n) HEA_HIP(s->gain.alloc(n));
-  s->guide.gain = s->gain.p;
-  s->guide.every = every;
+  const size_t n = (size_t)s->L * s->p->hc.N * 4;
+  if (s->hist.n < n) HEA_HIP(s->hist.alloc(n));
+  size_t lds;
+  bool sp;
+  if ((rc = mc_lds(s->p, s->task, &lds, &sp, s->tp))) return rc;
+  s->lds = lds;
+  s->sp_lds = sp;
+  s->guide.hist = s->hist.p;
   s->guide.cand = candidates;
   s->guide.tau = tau;
   if (s->exec) {  // the graph bakes the launch arguments in: capture again
@@ -976,9 +988,9 @@ struct hea_batch_s {
   DevBuf<int64_t> stats;
   DevBuf<uint32_t> iter, queue;
   DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
-  DevBuf<HeaMcArgs> margs;         // per-problem fields of the searched population B
+  DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
   DevBuf<int2> items;
-  DevBuf<float> gain;              // guided mode, [sum_p L N_p K_p]
+  DevBuf<uint32_t> hist;           // guided mode, [sum_p L N_p 4]
   GuideCfg guide;
 };
 
@@ -1000,21 +1012,11 @@ static int batch_enqueue_iteration(hea_batch b) {
       return rc;
   }
   HeaMcBatchArgs ba;
-  ba.probs = b->margs.p;
   ba.items = b->items.p;
   ba.n_items = P * L;
   ba.queue = b->queue.p;
-  ba.run.task = b->task;
-  ba.run.depth = b->depth;
-  ba.run.tp = b->tp;
-  ba.run.thr = b->thr;
-  ba.run.iter_ptr = b->iter.p;
-  ba.run.seed = b->seed;
-  ba.run.g_every = b->guide.every;
-  ba.run.g_cand = b->guide.cand;
-  ba.run.g_tau = b->guide.tau;
   for (int phase = HEA_PHASE_LS1; phase <= HEA_PHASE_LS2; ++phase) {
-    ba.run.phase = (uint32_t)phase;
+    ba.probs = b->margs.p + (size_t)(phase - HEA_PHASE_LS1) * P;
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, b->queue.p, 0u);
     HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->grid, b->threads, b->lds, st));
     for (int q = 0; q < P; ++q) {
@@ -1111,25 +1113,33 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (e == hipSuccess) e = b->iter.alloc(1);
   if (e == hipSuccess) e = b->queue.alloc(1);
   if (e == hipSuccess) e = b->clk.alloc((size_t)1 + n_probs);
-  if (e == hipSuccess) e = b->margs.alloc(n_probs);
+  if (e == hipSuccess) e = b->margs.alloc((size_t)2 * n_probs);
   if (e == hipSuccess) e = b->items.alloc(items.size());
   if (e != hipSuccess) {
     delete b;
     return fail(HEA_ENOMEM, std::string("batch allocation: ") + hipGetErrorString(e));
   }
-  std::vector<HeaMcArgs> ma(n_probs);
-  for (int q = 0; q < n_probs; ++q) {
-    HeaMcArgs& a = ma[q];
-    memset(&a, 0, sizeof(a));
-    fill_problem_args(probs[q], a);
-    a.lattices = b->B.p + b->lat_off[q];
-    a.fitness = b->fB.p + (size_t)q * L;
-    a.counts = b->counts.p + b->cnt_off[q];
-    a.stats = b->stats.p + (size_t)q * 4 * L;
-    a.first_hit = b->clk.p + 1 + q;
-    a.L = L;
-  }
-  e = hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * n_probs, hipMemcpyHostToDevice);
+  std::vector<HeaMcArgs> ma((size_t)2 * n_probs);
+  for (int ph = 0; ph < 2; ++ph)
+    for (int q = 0; q < n_probs; ++q) {
+      HeaMcArgs& a = ma[(size_t)ph * n_probs + q];
+      memset(&a, 0, sizeof(a));
+      fill_problem_args(probs[q], a);
+      a.lattices = b->B.p + b->lat_off[q];
+      a.fitness = b->fB.p + (size_t)q * L;
+      a.counts = b->counts.p + b->cnt_off[q];
+      a.stats = b->stats.p + (size_t)q * 4 * L;
+      a.first_hit = b->clk.p + 1 + q;
+      a.iter_ptr = b->iter.p;
+      a.L = L;
+      a.task = task;
+      a.depth = depth;
+      a.tp = b->tp;
+      a.thr = thr;
+      a.seed = seed + (uint64_t)q;
+      a.phase = (uint32_t)(HEA_PHASE_LS1 + ph);
+    }
+  e = hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * ma.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds);
@@ -1157,6 +1167,16 @@ int hea_batch_grid(hea_batch b, int32_t grid) {
   return HEA_OK;
 }
 
+// Rewrites the batch's device launch arguments with `patch` applied.
+static int batch_patch_args(hea_batch b, const std::function<void(HeaMcArgs&, int)>& patch) {
+  const size_t n = (size_t)2 * b->probs.size();
+  std::vector<HeaMcArgs> ma(n);
+  HEA_HIP(hipMemcpy(ma.data(), b->margs.p, sizeof(HeaMcArgs) * n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) patch(ma[i], (int)(i % b->probs.size()));
+  HEA_HIP(hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * n, hipMemcpyHostToDevice));
+  return HEA_OK;
+}
+
 int hea_batch_tile(hea_batch b, int32_t tp) {
   if (!b || !valid_tp(tp, b->S))
     return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256 with shells*tile <= 1024");
@@ -1172,6 +1192,8 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   }
   HEA_HIP(hipSetDevice(b->probs[0]->device));
   HEA_HIP(hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds));
+  int rc = batch_patch_args(b, [tp](HeaMcArgs& a, int) { a.tp = tp; });
+  if (rc) return rc;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
@@ -1181,24 +1203,28 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   return HEA_OK;
 }
 
-int hea_batch_guide(hea_batch b, int32_t every, int32_t candidates, double tau) {
-  if (!b || every < 1 || candidates < 1 || !(tau > 0.0))
-    return fail(HEA_EINVAL, "guide needs every >= 1, candidates >= 1, tau > 0");
+int hea_batch_guide(hea_batch b, int32_t candidates, double tau) {
+  if (!b) return fail(HEA_EINVAL, "null batch");
+  int rc;
+  for (auto q : b->probs)
+    if ((rc = check_guide(q, candidates, tau))) return rc;
   HEA_HIP(hipSetDevice(b->probs[0]->device));
   const int P = (int)b->probs.size();
   size_t n = 0;
   std::vector<size_t> off(P);
   for (int q = 0; q < P; ++q) {
     off[q] = n;
-    n += (size_t)b->L * b->probs[q]->hc.N * b->probs[q]->hc.K;
+    n += (size_t)b->L * b->probs[q]->hc.N * 4;
   }
-  if (b->gain.n < n) HEA_HIP(b->gain.alloc(n));
-  std::vector<HeaMcArgs> ma(P);
-  HEA_HIP(hipMemcpy(ma.data(), b->margs.p, sizeof(HeaMcArgs) * P, hipMemcpyDeviceToHost));
-  for (int q = 0; q < P; ++q) ma[q].gain = b->gain.p + off[q];
-  HEA_HIP(hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * P, hipMemcpyHostToDevice));
-  b->guide.gain = b->gain.p;
-  b->guide.every = every;
+  if (b->hist.n < n) HEA_HIP(b->hist.alloc(n));
+  uint32_t* base = b->hist.p;
+  rc = batch_patch_args(b, [&](HeaMcArgs& a, int q) {
+    a.hist = base + off[q];
+    a.g_cand = candidates;
+    a.g_tau = tau;
+  });
+  if (rc) return rc;
+  b->guide.hist = b->hist.p;
   b->guide.cand = candidates;
   b->guide.tau = tau;
   if (b->exec) {

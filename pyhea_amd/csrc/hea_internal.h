@@ -52,9 +52,9 @@ struct HeaMcArgs {
   int32_t L, task, depth, tp;   // tp: proposals per pass (lanes per shell)
   uint32_t phase;
   double thr;
-  // guided proposals (gain == nullptr: plain keyed stream)
-  float* gain;                  // [L][N][K] site gains, refreshed every g_every rounds
-  int32_t g_every, g_cand;      // refresh period (rounds), candidates per proposal
+  // guided proposals (hist == nullptr: plain keyed stream)
+  uint32_t* hist;               // [L][N][4] packed neighbour species counts per shell
+  int32_t g_cand;               // candidate pairs per proposal
   double g_tau;                 // softmax temperature in units of the gain scale
   // problem scalars, copied from HeaConst so they live in the kernarg segment
   int32_t N, K, S, stride;
@@ -62,17 +62,14 @@ struct HeaMcArgs {
   double delta[HEA_MAX_S], weight[HEA_MAX_S];
 };
 
-// One launch of the persistent local search over many problems.  probs[p]
-// holds problem p's population fields (pc, nbr, lattices, fitness, counts,
-// stats, first_hit, L and the problem scalars); run holds the shared run
-// fields (task, depth, tp, phase, thr, iter_ptr, seed).  items[i] =
-// (problem, lattice), processed in order.
+// One launch of the persistent local search over many problems: probs[p]
+// holds problem p's complete launch arguments; items[i] = (problem,
+// lattice), processed in order.
 struct HeaMcBatchArgs {
   const HeaMcArgs* probs;
   const int2* items;
   int32_t n_items;
   uint32_t* queue;  // zeroed before each launch
-  HeaMcArgs run;
 };
 
 // R_s(k) from the binade segments (exact: the host verified every k).

@@ -55,7 +55,7 @@ struct McShared {
   int wj[kMaxWaves];
   uint32_t next_pos;
   int pad;
-  double gscale;  // guided mode: scale of the site gains
+  unsigned gmax;  // guided mode: float bits of max_s Z_s |D| (gain scale)
 };
 
 // Species histogram of one shell walk, packed one field per species:
@@ -155,7 +155,8 @@ struct Lds {
   McShared* sh;
   uint8_t* sp;   // [N] when resident
   static __host__ __device__ size_t bytes(int task, int tp, int n, bool sp_lds) {
-    size_t b = sizeof(double) * (5 * TAB + KMAX + HEA_MAX_S * (size_t)tp);
+    size_t b = sizeof(double) * (5 * TAB + KMAX + HEA_MAX_S * (size_t)tp +
+                                  (HEA_MAX_S * KMAX * KMAX * KMAX + 1) / 2);
     b += sizeof(int2) * (size_t)task + sizeof(McShared);
     if (sp_lds) b += ((size_t)n + 15) & ~(size_t)15;
     return b;
@@ -165,8 +166,8 @@ struct Lds {
     E0 = G + TAB;
     R = E0 + TAB;
     tgt = R + TAB;
-    H = tgt + TAB;
-    conc = H + TAB;
+    H = tgt + TAB;  // followed by D: [S][KMAX][KMAX][KMAX] floats
+    conc = H + TAB + (HEA_MAX_S * KMAX * KMAX * KMAX + 1) / 2;
     esum = conc + KMAX;
     prop = reinterpret_cast<int2*>(esum + HEA_MAX_S * tp);
     sh = reinterpret_cast<McShared*>(prop + task);
@@ -282,54 +283,36 @@ __device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>
   return es;
 }
 
-// Guided mode: site gains u[k][c] = g[k][c] - g[k][sp[k]] of lattice l at
-// its current gamma, with g[k][c] = sum_s sum_b H_s[c][b] cnt_s[k][b] and
-// H_s = dF2/dn_s + (dF2/dn_s)^T, dF2/dn_sab = -2 w_s (alpha_sab - t_sab) /
-// (Z_s N c_a c_b) (the closed-form derivative of the reference fitness
-// squared, accelerate.cpp:556-569, at gamma = n / (Z_s N)).
+// Guided mode: H_s = dF2/dn_s + (dF2/dn_s)^T with the closed-form
+// dF2/dn_sab = -2 w_s (alpha_sab - t_sab) / (Z_s N c_a c_b) of the reference
+// fitness squared (accelerate.cpp:556-569) at gamma = n / (Z_s N); then
+// D[s][a][b][d] = H_s[b][d] - H_s[a][d], so that swapping i (species a) with
+// j (species b) changes F^2 by sum_s sum_d D[s][a][b][d] (cnt_s[i][d] -
+// cnt_s[j][d]) to first order (cnt: neighbour species counts).
 template <int KMAX>
-__device__ void refresh_gain(const HeaMcArgs& A, const Lds<KMAX>& T, const uint8_t* sp, int l) {
+__device__ void guide_tables(const HeaMcArgs& A, const Lds<KMAX>& T) {
   const int K = A.K, S = A.S, N = A.N, nthr = blockDim.x, tid = threadIdx.x;
   for (int e = tid; e < S * K * K; e += nthr) {
     const int s = e / (K * K), c = (e / K) % K, b = e % K;
     const double zn = (double)A.Z[s] * N;
     const double ccb = T.conc[c] * T.conc[b];
-    const double dcb = -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, c, b)] / ccb) - T.tgt[T.idx(s, c, b)]) /
-                       (zn * ccb);
-    const double dbc = -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, b, c)] / ccb) - T.tgt[T.idx(s, b, c)]) /
-                       (zn * ccb);
+    const double dcb =
+        -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, c, b)] / ccb) - T.tgt[T.idx(s, c, b)]) / (zn * ccb);
+    const double dbc =
+        -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, b, c)] / ccb) - T.tgt[T.idx(s, b, c)]) / (zn * ccb);
     T.H[T.idx(s, c, b)] = dcb + dbc;
   }
+  if (tid == 0) T.sh->gmax = 0u;
   __syncthreads();
-  if (tid == 0) {
-    double sc = 0.0;
-    for (int s = 0; s < S; ++s) {
-      double m = 0.0;
-      for (int c = 0; c < K; ++c)
-        for (int b = 0; b < K; ++b) m = fmax(m, fabs(T.H[T.idx(s, c, b)]));
-      sc += A.Z[s] * m;
-    }
-    T.sh->gscale = sc > 0.0 ? sc : 1.0;
+  float* D = reinterpret_cast<float*>(T.H + Lds<KMAX>::TAB);
+  unsigned mx = 0u;
+  for (int e = tid; e < S * K * K * K; e += nthr) {
+    const int s = e / (K * K * K), a = (e / (K * K)) % K, b = (e / K) % K, d = e % K;
+    const float v = (float)(T.H[T.idx(s, b, d)] - T.H[T.idx(s, a, d)]);
+    D[((s * KMAX + a) * KMAX + b) * KMAX + d] = v;
+    mx = max(mx, __float_as_uint(fabsf(v) * (float)A.Z[s]));
   }
-  float* u = A.gain + (size_t)l * N * K;
-  for (int k = tid; k < N; k += nthr) {
-    const int32_t* row = A.nbr + (size_t)k * A.stride;
-    double g[KMAX];
-    for (int c = 0; c < KMAX; ++c) g[c] = 0.0;
-    for (int s = 0; s < S; ++s) {
-      uint64_t m = 0;  // 8-bit species counts of this shell
-      for (int q = 0; q < A.Z[s]; ++q) m += 1ull << (8 * sp[row[A.off[s] + q]]);
-      for (int b = 0; b < K; ++b) {
-        const double cb = (double)((m >> (8 * b)) & 255u);
-        if (cb != 0.0)
-          for (int c = 0; c < K; ++c) g[c] += T.H[T.idx(s, c, b)] * cb;
-      }
-    }
-    const int a = sp[k];
-    double ga = 0.0;
-    for (int c = 0; c < K; ++c) ga = (c == a) ? g[c] : ga;
-    for (int c = 0; c < K; ++c) u[(size_t)k * K + c] = (float)(g[c] - ga);
-  }
+  atomicMax(&T.sh->gmax, mx);
   __syncthreads();
 }
 
@@ -385,39 +368,73 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   bool hit_thr = false;
   double nA[KMAX], nB[KMAX];  // this lane's shell rows after its last proposal
 
-  const bool guided = A.gain != nullptr;
-  if (guided) refresh_gain<KMAX>(A, T, sp, l);
+  const bool guided = A.hist != nullptr;
+  // guided mode: hist[k] = packed neighbour species counts of site k, one
+  // 32-bit word per shell (5-bit fields), 16 bytes per site
+  uint32_t* hist = guided ? A.hist + (size_t)l * N * 4 : nullptr;
+  if (guided) {
+    for (int k = tid; k < N; k += nthr) {
+      const int32_t* row = A.nbr + (size_t)k * A.stride;
+      uint32_t m[4] = {0u, 0u, 0u, 0u};
+      for (int s = 0; s < S; ++s)
+        for (int q = 0; q < A.Z[s]; ++q) m[s] += 1u << (5 * sp[row[A.off[s] + q]]);
+      *reinterpret_cast<uint4*>(hist + (size_t)k * 4) = make_uint4(m[0], m[1], m[2], m[3]);
+    }
+    guide_tables<KMAX>(A, T);
+  }
 
   while (acc < A.depth) {
     if (guided) {
       // ---- guided: each slot takes the Gumbel-max of -gain/(tau*scale)
       // over g_cand keyed candidate pairs (same-species candidates skipped)
-      if (rounds > 0 && rounds % A.g_every == 0) refresh_gain<KMAX>(A, T, sp, l);
-      const float* u = A.gain + (size_t)l * N * K;
-      const float inv = (float)(1.0 / (A.g_tau * T.sh->gscale));
+      const float* D = reinterpret_cast<const float*>(T.H + Lds<KMAX>::TAB);
+      const float gs = __uint_as_float(T.sh->gmax);
+      const float inv = (float)(1.0 / (A.g_tau * (gs > 0.0f ? (double)gs : 1.0)));
+      const int c4 = (A.g_cand + 3) & ~3;  // candidates per slot, in groups of 4
       for (int j = tid; j < task; j += nthr) {
         float best = -3.0e38f;
         int2 bp = make_int2(-1, -1);
-        for (int c = 0; c < A.g_cand; ++c) {
-          const uint32_t pi = pos + (uint32_t)(j * A.g_cand + c);
-          uint32_t s1, s2;
-          hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pi, (uint32_t)N, &s1, &s2);
-          const int a = sp[s1], b = sp[s2];
-          if (a != b) {
-            const hea_u32x4 w = hea_stream_block(A.seed, iter, A.phase | 0x100u, (uint32_t)l, pi >> 2);
-            const uint32_t wv = (pi & 3u) == 0 ? w.x : (pi & 3u) == 1 ? w.y : (pi & 3u) == 2 ? w.z : w.w;
-            const float U = ((float)wv + 0.5f) * (1.0f / 2147483648.0f);
-            const float sc = -(u[(size_t)s1 * K + b] + u[(size_t)s2 * K + a]) * inv -
-                             logf(-logf(U));
-            if (sc > best) {
-              best = sc;
-              bp = make_int2((int)s1, (int)s2);
+        for (int c0 = 0; c0 < A.g_cand; c0 += 4) {
+          const uint32_t pi0 = pos + (uint32_t)(j * c4 + c0);  // multiple of 4
+          const hea_u32x4 w01 = hea_stream_block(A.seed, iter, A.phase, (uint32_t)l, pi0 >> 1);
+          const hea_u32x4 w23 = hea_stream_block(A.seed, iter, A.phase, (uint32_t)l, (pi0 >> 1) + 1);
+          const hea_u32x4 nz = hea_stream_block(A.seed, iter, A.phase | 0x100u, (uint32_t)l, pi0 >> 2);
+          const uint32_t ws[8] = {w01.x, w01.y, w01.z, w01.w, w23.x, w23.y, w23.z, w23.w};
+          const uint32_t ns[4] = {nz.x, nz.y, nz.z, nz.w};
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if (c0 + c < A.g_cand) {
+              const uint32_t s1 = ws[2 * c] % (uint32_t)N, s2 = ws[2 * c + 1] % (uint32_t)N;
+              const int a = sp[s1], b = sp[s2];
+              if (a != b) {
+                const uint4 h1 = *reinterpret_cast<const uint4*>(hist + (size_t)s1 * 4);
+                const uint4 h2 = *reinterpret_cast<const uint4*>(hist + (size_t)s2 * 4);
+                const uint32_t hh1[3] = {h1.x, h1.y, h1.z}, hh2[3] = {h2.x, h2.y, h2.z};
+                float d = 0.0f;
+#pragma unroll
+                for (int s = 0; s < HEA_MAX_S; ++s) {
+                  if (s < S) {
+                    const float* Dr = D + ((s * KMAX + a) * KMAX + b) * KMAX;
+#pragma unroll
+                    for (int q = 0; q < KMAX; ++q)
+                      if (q < K)
+                        d += Dr[q] * (float)((int)((hh1[s] >> (5 * q)) & 31u) -
+                                             (int)((hh2[s] >> (5 * q)) & 31u));
+                  }
+                }
+                const float U = ((float)ns[c] + 0.5f) * (1.0f / 2147483648.0f);
+                const float sc = -d * inv - __logf(-__logf(U));
+                if (sc > best) {
+                  best = sc;
+                  bp = make_int2((int)s1, (int)s2);
+                }
+              }
             }
           }
         }
         T.prop[j] = bp;
       }
-      pos += (uint32_t)(task * A.g_cand);
+      pos += (uint32_t)(task * ((A.g_cand + 3) & ~3));
       __syncthreads();
     }
     // ---- draw `task` valid proposals in stream order (accelerate.cpp:649-655)
@@ -526,11 +543,30 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         sp[pr.x] = (uint8_t)b;
         sp[pr.y] = (uint8_t)a;
       }
+      if (guided) {
+        // neighbours of the swapped sites see a -> b (around pr.x) and
+        // b -> a (around pr.y) in their histograms
+        int zs = 0;
+        for (int s = 0; s < S; ++s) zs += A.Z[s];
+        for (int t = tid; t < 2 * zs; t += nthr) {
+          const int site = t < zs ? pr.x : pr.y;
+          int q = t < zs ? t : t - zs, s = 0;
+          while (q >= A.Z[s]) q -= A.Z[s++];
+          const int nb = A.nbr[(size_t)site * A.stride + A.off[s] + q];
+          const uint32_t dv = t < zs ? (1u << (5 * b)) - (1u << (5 * a)) : (1u << (5 * a)) - (1u << (5 * b));
+          atomicAdd(hist + (size_t)nb * 4 + s, dv);
+        }
+      }
       for (int e = tid; e < S * K * K; e += nthr) {
         const int s = e / (K * K), i = (e / K) % K, j = e % K;
         refresh_entry(A, T, s, i, j);
       }
       __syncthreads();
+      if (guided) {
+        // the histogram atomics went to L2: drop this CU's L1 copies
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        guide_tables<KMAX>(A, T);
+      }
       if (hit_thr && tid == 0 && A.first_hit) {
         atomicMin(A.first_hit, (unsigned long long)wall_clock64());
         hit_thr = false;
@@ -561,8 +597,8 @@ __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
 
 // Persistent local search over many problems: every workgroup takes
 // (problem, lattice) items from a device queue, longest first, until the
-// queue is empty.  Per-problem fields come from B.probs; run fields from
-// B.run.  Problem p draws from seed B.run.seed + p.
+// queue is empty.  B.probs[p] holds problem p's complete launch arguments
+// (its seed is the batch seed + p).
 template <int KMAX, bool SP_LDS>
 __global__ __launch_bounds__(1024) void k_local_mc_batch(HeaMcBatchArgs B) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -574,17 +610,7 @@ __global__ __launch_bounds__(1024) void k_local_mc_batch(HeaMcBatchArgs B) {
     __syncthreads();
     if (item >= B.n_items) break;
     const int2 it = B.items[item];
-    HeaMcArgs A = B.probs[it.x];
-    A.task = B.run.task;
-    A.depth = B.run.depth;
-    A.tp = B.run.tp;
-    A.phase = B.run.phase;
-    A.thr = B.run.thr;
-    A.iter_ptr = B.run.iter_ptr;
-    A.seed = B.run.seed + (uint64_t)it.x;
-    A.g_every = B.run.g_every;
-    A.g_cand = B.run.g_cand;
-    A.g_tau = B.run.g_tau;
+    const HeaMcArgs& A = B.probs[it.x];
     mc_lattice<KMAX, SP_LDS, false, false>(A, it.y, smem);
     __syncthreads();
   }

@@ -137,9 +137,9 @@ class Session:
         """Proposals evaluated per pass (launch shape only)."""
         N.check(N.lib().hea_session_tile(self._h, int(tp)))
 
-    def guide(self, every: int = 16, candidates: int = 16, tau: float = 0.1) -> None:
+    def guide(self, candidates: int = 8, tau: float = 0.1) -> None:
         """Switch both local searches to gradient-guided proposals."""
-        N.check(N.lib().hea_session_guide(self._h, every, candidates, tau))
+        N.check(N.lib().hea_session_guide(self._h, candidates, tau))
 
     def init(self) -> None:
         N.check(N.lib().hea_session_init(self._h))
@@ -198,8 +198,8 @@ class Batch:
                                          converge_depth, thr, seed, C.byref(h)))
         self._h = h
 
-    def guide(self, every: int = 16, candidates: int = 16, tau: float = 0.1) -> None:
-        N.check(N.lib().hea_batch_guide(self._h, every, candidates, tau))
+    def guide(self, candidates: int = 8, tau: float = 0.1) -> None:
+        N.check(N.lib().hea_batch_guide(self._h, candidates, tau))
 
     def tile(self, tp: int) -> None:
         N.check(N.lib().hea_batch_tile(self._h, int(tp)))

@@ -20,7 +20,7 @@ def _spec(typ="FCC", cell=4, K=4, seed=11):
 def _run(prob, guided, L=8, I=3, T=64, D=3, seed=5):
     s = prob.session(L, T, D, seed=seed)
     if guided:
-        s.guide(every=4, candidates=8, tau=0.1)
+        s.guide(candidates=8, tau=0.1)
     s.init()
     s.step(I)
     out = s.read()
@@ -46,12 +46,12 @@ def test_guided_batch_matches_guided_sessions(gpu):
     specs = [_spec("FCC", 3, 4, 21), _spec("BCC", 4, 5, 22)]
     probs = [Problem.from_spec(s) for s in specs]
     b = Batch(probs, 4, 32, 2, seed=70)
-    b.guide(every=4, candidates=8, tau=0.1)
+    b.guide(candidates=8, tau=0.1)
     b.init()
     b.step(2)
     for q, p in enumerate(probs):
         s = p.session(4, 32, 2, seed=70 + q)
-        s.guide(every=4, candidates=8, tau=0.1)
+        s.guide(candidates=8, tau=0.1)
         s.init()
         s.step(2)
         ls, fs, _ = s.read()
