@@ -1018,7 +1018,8 @@ static int batch_enqueue_iteration(hea_batch b) {
   for (int phase = HEA_PHASE_LS1; phase <= HEA_PHASE_LS2; ++phase) {
     ba.probs = b->margs.p + (size_t)(phase - HEA_PHASE_LS1) * P;
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, b->queue.p, 0u);
-    HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->grid, b->threads, b->lds, st));
+    HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->guide.hist != nullptr, b->grid,
+                                      b->threads, b->lds, st));
     for (int q = 0; q < P; ++q) {
       hea_problem p = b->probs[q];
       const int N = p->hc.N;

@@ -87,5 +87,5 @@ hipError_t hea_prepare_local_mc(int K, bool sp_lds, bool self, size_t lds);
 hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self, size_t lds,
                                hipStream_t st);
 hipError_t hea_prepare_local_mc_batch(int kmax, bool sp_lds, size_t lds);
-hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_lds, int grid,
-                                     int threads, size_t lds, hipStream_t st);
+hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_lds, bool guided,
+                                     int grid, int threads, size_t lds, hipStream_t st);

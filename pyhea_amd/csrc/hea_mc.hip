@@ -12,14 +12,25 @@
 //     drifted gamma forward (accelerate.cpp:682);
 //   * the search ends after `depth` consecutive rejected rounds.
 //
-// Work layout.  A workgroup has S x TP lanes: lane (s, j) evaluates shell s
-// of proposal j (TP = proposals per pass, a multiple of 64, so every wave
-// holds one shell and walks neighbour lists of one uniform length).  Each
-// lane produces its shell's partial sum e_s = sum_ij (alpha_sij - t_sij)^2 in
-// the reference's (i, j) order; the shell-0 lane then forms
-// err = ((0 + w0 e0) + w1 e1) + w2 e2 and F = sqrt(err) exactly as
-// accelerate.cpp:557-569 does.  The lattice's species sit in LDS when they
-// fit, otherwise they are read from HBM.
+// Work layout.  A workgroup has S x TP lanes: lane (s, j) handles shell s of
+// proposal j (TP = proposals per pass, a multiple of 64, so every wave holds
+// one shell and walks neighbour lists of one uniform length).  A round has
+// two stages:
+//   1. screen: every lane walks its proposal's two neighbour lists (species
+//      histograms MA, MB and adjacency counts), keeps them in LDS and forms a
+//      division-free estimate x_s of the shell's change of the fitness
+//      squared; X = sum_s w_s x_s.
+//   2. exact: wave 0 takes every proposal whose X lies within 2*beta of the
+//      smallest X (normally one) and evaluates it bit-exactly as the
+//      reference does: the shell's partial sum e_s = sum_ij (alpha_sij -
+//      t_sij)^2 in the reference's (i, j) order, err = ((0 + w0 e0) + w1 e1)
+//      + w2 e2 and F = sqrt(err) (accelerate.cpp:557-569).  The first strict
+//      minimum of F over these candidates is the reference's choice, because
+//      beta bounds |err - (E + X)| for every proposal (E common to all) plus a
+//      margin that keeps F ties inside the window (proof beside
+//      refresh_entry).
+// The lattice's species sit in LDS when they fit, otherwise they are read
+// from HBM.
 //
 // Exact FP64 bookkeeping.  The reference updates gamma by +-delta_s with
 // delta_s = 1/(Z_s*N) (accelerate.cpp:526-548); the result depends on the
@@ -39,7 +50,8 @@
 // values is not provably inside one binade, or whose step ties, are marked
 // r = 0 and replayed addition by addition.  Either way the result equals the
 // reference's doubles bit for bit.  No FP contraction is allowed in this file
-// (-ffp-contract=off).
+// (-ffp-contract=off); the screen uses explicit fma where its error bound
+// allows.
 
 #include <type_traits>
 
@@ -48,13 +60,17 @@
 namespace {
 
 constexpr int kMaxWaves = 16;  // 1024 lanes
+constexpr double kU = 0x1p-53;  // unit roundoff of binary64
 
 struct McShared {
   int wcount[kMaxWaves];
-  unsigned long long wkey[kMaxWaves];
-  int wj[kMaxWaves];
+  double wmin[kMaxWaves];                // per-wave smallest screen value
+  unsigned long long bmax[HEA_MAX_S];   // bits of the largest per-entry screen error bound
+  unsigned long long ubmax[HEA_MAX_S];  // bits of the largest bound on |d'| (see refresh_entry)
+  double beta;                          // candidate window half-width of this round
+  unsigned long long res_key;           // exact stage: winner's fitness bits
+  int res_j, res_a, res_b;
   uint32_t next_pos;
-  int pad;
   unsigned gmax;  // guided mode: float bits of max_s Z_s |D| (gain scale)
 };
 
@@ -140,147 +156,240 @@ __device__ __forceinline__ void walk(const int32_t* __restrict__ row, int z,
   }
 }
 
-// LDS carve-up of one workgroup.
+// LDS carve-up of one workgroup (byte offsets; every region 16-byte aligned).
+template <int KMAX>
+struct LdsLayout {
+  static constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
+  using PT = typename Pack<KMAX>::T;
+  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, WA, WB, WC, cl, sh, sp, total;
+  __host__ __device__ LdsLayout(int task, int n, bool sp_lds) {
+    size_t o = 0;
+    auto take = [&o](size_t bytes) {
+      const size_t r = o;
+      o = (o + bytes + 15) & ~(size_t)15;
+      return r;
+    };
+    G = take(sizeof(double) * TAB);
+    E0 = take(sizeof(double) * TAB);
+    R = take(sizeof(double) * TAB);
+    tgt = take(sizeof(double) * TAB);
+    SC = take(sizeof(double2) * TAB);
+    H = take(sizeof(double) * TAB + sizeof(float) * HEA_MAX_S * KMAX * KMAX * KMAX);
+    conc = take(sizeof(double) * KMAX);
+    xs = take(sizeof(double) * HEA_MAX_S * (size_t)task);
+    ex = take(sizeof(double) * 64);
+    xw = take(sizeof(double) * 64 * (4 + KMAX));
+    prop = take(sizeof(int2) * (size_t)task);
+    WA = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
+    WB = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
+    WC = take(sizeof(uint32_t) * HEA_MAX_S * (size_t)task);
+    cl = take(sizeof(int) * (size_t)task);
+    sh = take(sizeof(McShared));
+    sp = o;
+    total = sp_lds ? ((o + (size_t)n + 15) & ~(size_t)15) : o;
+  }
+};
+
 template <int KMAX>
 struct Lds {
-  static constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
+  static constexpr int TAB = LdsLayout<KMAX>::TAB;
+  using PT = typename Pack<KMAX>::T;
   double* G;     // [S][KMAX][KMAX] prev_gamma
   double* E0;    // [S][KMAX][KMAX] (1 - G/d - t)^2 of prev_gamma
   double* R;     // [S][KMAX][KMAX] safe step or 0
   double* tgt;   // [S][KMAX][KMAX] target alpha
-  double* H;     // [S][KMAX][KMAX] guided mode: dF2/dn + its transpose
+  double2* SC;   // [S][KMAX][KMAX] {d_ij + d_ji, -delta_s/dd_ij}, d = alpha - t
+  double* H;     // [S][KMAX][KMAX] guided mode: dF2/dn + its transpose; then D floats
   double* conc;  // [KMAX] fl(count/N)
-  double* esum;  // [S][TP] per-shell partial sums
+  double* xs;    // [S][task] screen values; row 0 then holds the combined X
+  double* ex;    // [64] exact-stage shell sums (wave 0)
+  double* xw;    // [64][4 + KMAX] exact stage, lane (c, s, x): gamma'(a,x), gamma'(b,x),
+                 // their alphas, then the terms of row x
   int2* prop;    // [task]
+  PT* WA;        // [S][task] walk results: histogram of a1's neighbours
+  PT* WB;        // [S][task] histogram of a2's neighbours
+  uint32_t* WC;  // [S][task] c11 | c12 << 8 | c21 << 16 | c22 << 24
+  int* cl;       // [task] exact-stage candidate list
   McShared* sh;
   uint8_t* sp;   // [N] when resident
-  static __host__ __device__ size_t bytes(int task, int tp, int n, bool sp_lds) {
-    size_t b = sizeof(double) * (5 * TAB + KMAX + HEA_MAX_S * (size_t)tp +
-                                  (HEA_MAX_S * KMAX * KMAX * KMAX + 1) / 2);
-    b += sizeof(int2) * (size_t)task + sizeof(McShared);
-    if (sp_lds) b += ((size_t)n + 15) & ~(size_t)15;
-    return b;
-  }
-  __device__ void carve(unsigned char* base, int task, int tp, bool sp_lds) {
-    G = reinterpret_cast<double*>(base);
-    E0 = G + TAB;
-    R = E0 + TAB;
-    tgt = R + TAB;
-    H = tgt + TAB;  // followed by D: [S][KMAX][KMAX][KMAX] floats
-    conc = H + TAB + (HEA_MAX_S * KMAX * KMAX * KMAX + 1) / 2;
-    esum = conc + KMAX;
-    prop = reinterpret_cast<int2*>(esum + HEA_MAX_S * tp);
-    sh = reinterpret_cast<McShared*>(prop + task);
-    sp = sp_lds ? reinterpret_cast<uint8_t*>(sh + 1) : nullptr;
+  static size_t bytes(int task, int n, bool sp_lds) { return LdsLayout<KMAX>(task, n, sp_lds).total; }
+  __device__ void carve(unsigned char* base, int task, bool sp_lds) {
+    const LdsLayout<KMAX> L(task, 0, sp_lds);
+    G = reinterpret_cast<double*>(base + L.G);
+    E0 = reinterpret_cast<double*>(base + L.E0);
+    R = reinterpret_cast<double*>(base + L.R);
+    tgt = reinterpret_cast<double*>(base + L.tgt);
+    SC = reinterpret_cast<double2*>(base + L.SC);
+    H = reinterpret_cast<double*>(base + L.H);
+    conc = reinterpret_cast<double*>(base + L.conc);
+    xs = reinterpret_cast<double*>(base + L.xs);
+    ex = reinterpret_cast<double*>(base + L.ex);
+    xw = reinterpret_cast<double*>(base + L.xw);
+    prop = reinterpret_cast<int2*>(base + L.prop);
+    WA = reinterpret_cast<PT*>(base + L.WA);
+    WB = reinterpret_cast<PT*>(base + L.WB);
+    WC = reinterpret_cast<uint32_t*>(base + L.WC);
+    cl = reinterpret_cast<int*>(base + L.cl);
+    sh = reinterpret_cast<McShared*>(base + L.sh);
+    sp = sp_lds ? base + L.sp : nullptr;
   }
   __device__ __forceinline__ int idx(int s, int i, int j) const { return (s * KMAX + i) * KMAX + j; }
 };
 
+// Orders LDS traffic between the lanes of one wave (the hardware keeps a
+// wave's LDS operations in order; this stops the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Refreshes the tables of entry (s,i,j) from prev_gamma and records its
+// screen error bounds.
+//
+// Screen error bound.  Write q = G/dd, d = alpha - t with the reference's
+// roundings (d-hat below), and for a proposal P let Delta = -(G' - G)/dd be
+// the exact change of alpha of an entry, |Delta| <= A (a net count of at
+// most 6 Z_s steps of delta_s; at most 6 Z_s additions, each rounding by at
+// most 2^-52 since |gamma| < 2).  The reference's
+// d-hat' - d-hat differs from Delta by e1 <= u (m + m'), m = |q| + |alpha| +
+// |d| (one rounding each in q, alpha and d); the screen's
+// Delta~ = fl(net * fl(-delta/dd)) differs from Delta by e2 <= (12 Z u delta
+// + (6Z + 4) 2^-52) / dd.  The true change of the entry pair's two terms,
+// fl(d'^2) - fl(d^2) summed over (i,j) and (j,i), then differs from the
+// screen's 2 Delta~ (D + Delta~), D = fl(d_ij + d_ji), by at most
+//   P = 2 e2 (D1 + A) + 2 (A + e2)(u D1 + e2) + 2 e1 (D1 + 2A + e1)
+//       + 2 u (D1 + A + e1)^2 + 2 u (A + e2)(D1 + A + e2),   D1 = |d_ij|+|d_ji|
+// (diagonal entries: the same with D1 = 2|d_ii|).  A proposal changes at most
+// 2K-1 such pairs per shell.  Adding the screen's own summation rounding and
+// the reference's rounding of its K*K-term sums (relative (K*K + S + 2) u of
+// err <= E_up = sum_s |w_s| K^2 ub_s^2, ub_s >= |d'| of every entry) bounds
+// |err(P) - (E + X(P))| by beta0 for one E common to all proposals.  With
+// beta = beta0 + 4 u E_up, any proposal with X > Xmin + 2 beta has
+// err > err(Pmin)(1 + 8u), hence sqrt(err) rounds strictly above the
+// minimum's F and cannot be the reference's choice.
 template <int KMAX>
 __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int i, int j) {
   const int e = T.idx(s, i, j);
   const double g = T.G[e];
-  const double d = T.conc[i] * T.conc[j];
-  const double sro = 1.0 - g / d;
-  const double df = sro - T.tgt[e];
-  T.E0[e] = df * df;
+  const double dd = T.conc[i] * T.conc[j];
+  const double q = g / dd;
+  const double al = 1.0 - q;
+  const double dij = al - T.tgt[e];
+  const double dji = al - T.tgt[T.idx(s, j, i)];
+  T.E0[e] = dij * dij;
   T.R[e] = safe_step(g, A.delta[s], A.Z[s]);
+  T.SC[e].x = dij + dji;
+  // error bounds (each computed with a few roundings; the 1.0001 factors
+  // cover them)
+  const double z = (double)A.Z[s], delta = A.delta[s];
+  const double D1 = fabs(dij) + fabs(dji);
+  const double Ae = (6.0 * z * delta + (6.0 * z + 4.0) * 0x1p-52) / dd * 1.0001;
+  const double e2 = (12.0 * z * kU * delta + (6.0 * z + 4.0) * 0x1p-52) / dd * 1.0001;
+  const double m = 2.0 * fabs(q) + 2.0 + D1 + 3.0 * Ae;
+  const double e1 = 2.0 * kU * m * 1.0001;
+  const double t1 = D1 + Ae + e1;
+  const double P = 2.0 * e2 * (D1 + Ae) + 2.0 * (Ae + e2) * (kU * D1 + e2) +
+                   2.0 * e1 * (D1 + 2.0 * Ae + e1) + 2.0 * kU * t1 * t1 +
+                   2.0 * kU * (Ae + e2) * (D1 + Ae + e2);
+  const double ub = D1 + Ae + e1 + e2;
+  atomicMax(&T.sh->bmax[s], (unsigned long long)__double_as_longlong(P * 1.0001));
+  atomicMax(&T.sh->ubmax[s], (unsigned long long)__double_as_longlong(ub * 1.0001));
 }
 
-// Shell s of the incremental fitness of swapping a1 <-> a2 (species a != b)
-// against prev_gamma.  Returns the shell's partial sum and fills the swapped
-// gamma rows a (nA) and b (nB) of that shell.
-template <int KMAX, bool EXACT, bool SELF>
-__device__ __forceinline__ double eval_shell(const HeaMcArgs& A, const Lds<KMAX>& T,
-                                             const uint8_t* __restrict__ sp, int s, int a1,
-                                             int a2, double (&nA)[KMAX], double (&nB)[KMAX]) {
+// beta of the refreshed tables (see refresh_entry).
+template <int KMAX>
+__device__ double screen_beta(const HeaMcArgs& A, const Lds<KMAX>& T) {
+  const double K = (double)A.K, S = (double)A.S;
+  double bs = 0.0, eup = 0.0;
+  for (int s = 0; s < A.S; ++s) {
+    const double bm = __longlong_as_double((long long)T.sh->bmax[s]);
+    const double ub = __longlong_as_double((long long)T.sh->ubmax[s]);
+    const double w = fabs(A.weight[s]);
+    // pair errors, then the screen's fma chain and shell/weight combination
+    // (|each screen term| <= 4 ub^2)
+    bs += w * ((2.0 * K) * bm + (2.0 * K + S + 2.0) * kU * (2.0 * K) * 4.0 * ub * ub);
+    eup += w * (K * K) * ub * ub;
+  }
+  eup *= 1.01;
+  return (bs + (K * K + S + 2.0) * kU * eup + 4.0 * kU * eup) * 1.01;
+}
+
+// Screen of shell s for swapping species a (at a1) and b (at a2): the
+// division-free change of the shell's sum of squares, sum over the changed
+// entry pairs of 2 Delta~ (D + Delta~) (diagonal entries once).
+template <int KMAX, bool EXACT>
+__device__ __forceinline__ double screen_shell(const Lds<KMAX>& T, int K, int s, int a, int b,
+                                               typename Pack<KMAX>::T MA,
+                                               typename Pack<KMAX>::T MB, int c11, int c12,
+                                               int c21, int c22) {
   using PK = Pack<KMAX>;
-  const int K = EXACT ? KMAX : A.K;
-  const int z = A.Z[s];
-  const double delta = A.delta[s];
-  const int a = sp[a1], b = sp[a2];
-  const int32_t* r1 = A.nbr + (size_t)a1 * A.stride + A.off[s];
-  const int32_t* r2 = A.nbr + (size_t)a2 * A.stride + A.off[s];
-  typename PK::T MA, MB;
-  int c11, c12, c22, c21;
-  walk<KMAX, SELF>(r1, z, sp, a1, a2, MA, c11, c12);
-  walk<KMAX, SELF>(r2, z, sp, a2, a1, MB, c22, c21);
   const int maA = PK::fld(MA, a), maB = PK::fld(MA, b);
   const int mbA = PK::fld(MB, a), mbB = PK::fld(MB, b);
+  const int nAA = 2 * (mbA - c21 + c22) - 2 * maA;
+  const int nBB = 2 * (maB - c12 + c11) - 2 * mbB;
+  const int nAB = (maA + c12 - c11) + (mbB + c21 - c22) - maB - mbA;
+  const double2* ra = T.SC + T.idx(s, a, 0);
+  const double2* rb = T.SC + T.idx(s, b, 0);
+  double x = 0.0;
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) {
+    if (EXACT || q < K) {
+      const int dm = PK::fld(MB, q) - PK::fld(MA, q);
+      const int na = (q == a) ? nAA : (q == b) ? nAB : dm;
+      const int nb = (q == b) ? nBB : -dm;
+      const double2 ea = ra[q];
+      const double2 eb = rb[q];
+      const double ga = (double)na * ea.y;
+      const double gb = (double)nb * eb.y;
+      // (b,a) is the mirror of (a,b): row b skips it
+      const double fa = (q == a) ? ga : 2.0 * ga;
+      const double fb = (q == a) ? 0.0 : (q == b) ? gb : 2.0 * gb;
+      x = fma(fa, ea.x + ga, x);
+      x = fma(fb, eb.x + gb, x);
+    }
+  }
+  return x;
+}
 
-  // fast path for every touched entry; collect the ones needing replay
-  const int eAA = T.idx(s, a, a), eBB = T.idx(s, b, b), eAB = T.idx(s, a, b);
+// Exact new values of gamma entries (a,x) and (b,x) of shell s for swapping
+// a1 <-> a2 (species a != b), from the stored walk of the two neighbour
+// lists (file header: fast path, or replay when the entry's step is 0).
+template <int KMAX>
+__device__ __forceinline__ void exact_entries(const HeaMcArgs& A, const Lds<KMAX>& T,
+                                              const uint8_t* __restrict__ sp, int s, int a1,
+                                              int a2, int a, int b, int x,
+                                              typename Pack<KMAX>::T MA,
+                                              typename Pack<KMAX>::T MB, int c11, int c12,
+                                              int c21, int c22, double& nA, double& nB) {
+  using PK = Pack<KMAX>;
+  const double delta = A.delta[s];
+  const int maA = PK::fld(MA, a), maB = PK::fld(MA, b);
+  const int mbA = PK::fld(MB, a), mbB = PK::fld(MB, b);
+  const int max_ = PK::fld(MA, x), mbx = PK::fld(MB, x);
   const int pAA = 2 * (mbA - c21 + c22), mAA = 2 * maA;
   const int pBB = 2 * (maB - c12 + c11), mBB = 2 * mbB;
   const int netAB = (maA + c12 - c11) + (mbB + c21 - c22) - maB - mbA;
-  const double gAA = T.G[eAA], rAA = T.R[eAA];
-  const double gBB = T.G[eBB], rBB = T.R[eBB];
-  const double gAB = T.G[eAB], rAB = T.R[eAB];
-  double vAA = fma((double)(pAA - mAA), rAA, gAA);
-  double vBB = fma((double)(pBB - mBB), rBB, gBB);
-  double vAB = fma((double)netAB, rAB, gAB);
-  unsigned need = (rAA == 0.0 ? 1u : 0u) | (rBB == 0.0 ? 2u : 0u) | (rAB == 0.0 ? 4u : 0u);
-  int ma[KMAX], mb[KMAX];
-#pragma unroll
-  for (int x = 0; x < KMAX; ++x) {
-    ma[x] = PK::fld(MA, x);
-    mb[x] = PK::fld(MB, x);
-    if (EXACT || x < K) {
-      const int eA = T.idx(s, a, x), eB = T.idx(s, b, x);
-      const double gA = T.G[eA], rA = T.R[eA];
-      const double gB = T.G[eB], rB = T.R[eB];
-      nA[x] = fma((double)(mb[x] - ma[x]), rA, gA);
-      nB[x] = fma((double)(ma[x] - mb[x]), rB, gB);
-      const bool gen = (x != a) && (x != b);
-      need |= (gen && rA == 0.0) ? (8u << x) : 0u;
-      need |= (gen && rB == 0.0) ? (2048u << x) : 0u;
-    }
+  const int eA = T.idx(s, a, x), eB = T.idx(s, b, x);
+  const double gA = T.G[eA], rA = T.R[eA];
+  const double gB = T.G[eB], rB = T.R[eB];
+  // (a,a) and (b,a) = (a,b) for x = a; (a,b) and (b,b) for x = b
+  const int netA = (x == a) ? pAA - mAA : (x == b) ? netAB : mbx - max_;
+  const int netB = (x == a) ? netAB : (x == b) ? pBB - mBB : max_ - mbx;
+  nA = fma((double)netA, rA, gA);
+  nB = fma((double)netB, rB, gB);
+  if (rA == 0.0 || rB == 0.0) {  // rare: near a binade edge, tied step or zero count
+    const int32_t* r1 = A.nbr + (size_t)a1 * A.stride + A.off[s];
+    const int32_t* r2 = A.nbr + (size_t)a2 * A.stride + A.off[s];
+    if (rA == 0.0)
+      nA = (x == a)   ? replay(gA, -delta, mAA, pAA)
+           : (x == b) ? replay_ab(gA, delta, r1, r2, A.Z[s], sp, a1, a2, a, b)
+                      : replay(gA, -delta, max_, mbx);
+    if (rB == 0.0)
+      nB = (x == a)   ? replay_ab(gB, delta, r1, r2, A.Z[s], sp, a1, a2, a, b)
+           : (x == b) ? replay(gB, delta, pBB, mBB)
+                      : replay(gB, delta, max_, mbx);
   }
-  if (need) {  // rare: entries near a binade edge, tied steps or zero counts
-    if (need & 1u) vAA = replay(gAA, -delta, mAA, pAA);
-    if (need & 2u) vBB = replay(gBB, delta, pBB, mBB);
-    if (need & 4u) vAB = replay_ab(gAB, delta, r1, r2, z, sp, a1, a2, a, b);
-#pragma unroll
-    for (int x = 0; x < KMAX; ++x) {
-      if (need & (8u << x)) nA[x] = replay(T.G[T.idx(s, a, x)], -delta, ma[x], mb[x]);
-      if (need & (2048u << x)) nB[x] = replay(T.G[T.idx(s, b, x)], delta, ma[x], mb[x]);
-    }
-  }
-
-  const double conc_a = T.conc[a], conc_b = T.conc[b];
-  double sA[KMAX], sB[KMAX];
-#pragma unroll
-  for (int x = 0; x < KMAX; ++x) {
-    if (EXACT || x < K) {
-      nA[x] = (x == a) ? vAA : (x == b) ? vAB : nA[x];
-      nB[x] = (x == b) ? vBB : (x == a) ? vAB : nB[x];
-      sA[x] = 1.0 - nA[x] / (conc_a * T.conc[x]);
-      sB[x] = 1.0 - nB[x] / (conc_b * T.conc[x]);
-    }
-  }
-
-  // all K*K terms in row-major order (accelerate.cpp:560-566)
-  const double* tg = T.tgt + T.idx(s, 0, 0);
-  const double* e0 = T.E0 + T.idx(s, 0, 0);
-  double es = 0.0;
-#pragma unroll
-  for (int i = 0; i < KMAX; ++i) {
-    if (EXACT || i < K) {
-      const bool ia = (i == a), ib = (i == b);
-#pragma unroll
-      for (int j = 0; j < KMAX; ++j) {
-        if (EXACT || j < K) {
-          const bool ja = (j == a), jb = (j == b);
-          const double sro = ia ? sA[j] : ib ? sB[j] : ja ? sA[i] : sB[i];
-          const double df = sro - tg[i * KMAX + j];
-          const double et = df * df;
-          const double e = (ia || ib || ja || jb) ? et : e0[i * KMAX + j];
-          es = es + e;
-        }
-      }
-    }
-  }
-  return es;
 }
 
 // Guided mode: H_s = dF2/dn_s + (dF2/dn_s)^T with the closed-form
@@ -321,21 +430,23 @@ __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned
   return (f1 < f2) || (f1 == f2 && j1 < j2);
 }
 
-// Local search of lattice l of the population described by A.
-template <int KMAX, bool SP_LDS, bool EXACT, bool SELF>
+// Local search of lattice l of the population described by A.  GUIDED
+// selects the proposal source (A.hist != nullptr): a separate instantiation,
+// so the plain kernels carry none of the guided code's registers.
+template <int KMAX, bool SP_LDS, bool EXACT, bool SELF, bool GUIDED>
 __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned char* smem) {
+  using PT = typename Pack<KMAX>::T;
   const HeaConst& P = *A.pc;
   const int nthr = blockDim.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int N = A.N, K = A.K, S = A.S;
   const int task = A.task, TP = A.tp;
-  const int nwaves = nthr >> 6;
   const int s_me = __builtin_amdgcn_readfirstlane(tid / TP);  // this wave's shell
   const int jl = tid - s_me * TP;                             // proposal slot in a pass
   const int nq = (task + TP - 1) / TP;                        // passes per round
 
   Lds<KMAX> T;
-  T.carve(smem, task, TP, SP_LDS);
+  T.carve(smem, task, SP_LDS);
   uint8_t* lat_g = A.lattices + (size_t)l * N;
   uint8_t* sp;
   if constexpr (SP_LDS) {
@@ -351,8 +462,17 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     const int s = e / (K * K), i = (e / K) % K, j = e % K;
     T.G[T.idx(s, i, j)] = hea_r_of_count(P, s, cnt[e]);
     T.tgt[T.idx(s, i, j)] = P.target[s][i][j];
+    T.SC[T.idx(s, i, j)].y = -A.delta[s] / (P.conc[i] * P.conc[j]);
   }
   if (tid < K) T.conc[tid] = P.conc[tid];
+  if (tid < HEA_MAX_S) {
+    T.sh->bmax[tid] = 0ull;
+    T.sh->ubmax[tid] = 0ull;
+  }
+  if (tid < kMaxWaves) {  // slots of absent waves stay neutral
+    T.sh->wcount[tid] = 0;
+    T.sh->wmin[tid] = __builtin_inf();
+  }
   __syncthreads();
   for (int e = tid; e < S * K * K; e += nthr) {
     const int s = e / (K * K), i = (e / K) % K, j = e % K;
@@ -366,13 +486,12 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   int acc = 0;
   long long rounds = 0, accepts = 0;
   bool hit_thr = false;
-  double nA[KMAX], nB[KMAX];  // this lane's shell rows after its last proposal
 
-  const bool guided = A.hist != nullptr;
+  constexpr bool guided = GUIDED;
   // guided mode: hist[k] = packed neighbour species counts of site k, one
   // 32-bit word per shell (5-bit fields), 16 bytes per site
   uint32_t* hist = guided ? A.hist + (size_t)l * N * 4 : nullptr;
-  if (guided) {
+  if constexpr (GUIDED) {
     for (int k = tid; k < N; k += nthr) {
       const int32_t* row = A.nbr + (size_t)k * A.stride;
       uint32_t m[4] = {0u, 0u, 0u, 0u};
@@ -384,7 +503,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   }
 
   while (acc < A.depth) {
-    if (guided) {
+    // window of this round's tables; published by the barriers below
+    if (tid == 0) T.sh->beta = screen_beta<KMAX>(A, T);
+    if constexpr (GUIDED) {
       // ---- guided: each slot takes the Gumbel-max of -gain/(tau*scale)
       // over g_cand keyed candidate pairs (same-species candidates skipped)
       const float* D = reinterpret_cast<const float*>(T.H + Lds<KMAX>::TAB);
@@ -449,7 +570,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       if (lane == 0) T.sh->wcount[wave] = __popcll(bal);
       __syncthreads();
       int off = 0, tot = 0;
-      for (int w = 0; w < nwaves; ++w) {
+#pragma unroll
+      for (int w = 0; w < kMaxWaves; ++w) {
         const int c = T.sh->wcount[w];
         off += (w < wave) ? c : 0;
         tot += c;
@@ -463,87 +585,217 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       if (found >= task) pos = T.sh->next_pos;
     }
 
-    // ---- evaluate; keep the first strict minimum (accelerate.cpp:663-674)
-    unsigned long long bkey = ~0ull;
-    int bj = 0x7fffffff;
+    // ---- screen: walk both neighbour lists of every proposal, keep the walk
     for (int q = 0; q < nq; ++q) {
       const int j = jl + q * TP;
-      if (j < task && T.prop[j].x >= 0) {
+      if (j < task) {
         const int2 pr = T.prop[j];
-        T.esum[s_me * TP + jl] =
-            eval_shell<KMAX, EXACT, SELF>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
-      }
-      __syncthreads();
-      if (s_me == 0 && j < task && T.prop[j].x >= 0) {
-        double err = 0.0;
-        for (int s = 0; s < S; ++s) err = err + A.weight[s] * T.esum[s * TP + jl];
-        const unsigned long long fk = __double_as_longlong(sqrt(err));
-        if (fk < bkey) {
-          bkey = fk;
-          bj = j;
+        double x = 0.0;
+        if (pr.x >= 0) {
+          const int a = sp[pr.x], b = sp[pr.y];
+          const int32_t* r1 = A.nbr + (size_t)pr.x * A.stride + A.off[s_me];
+          const int32_t* r2 = A.nbr + (size_t)pr.y * A.stride + A.off[s_me];
+          const int z = A.Z[s_me];
+          PT MA, MB;
+          int c11, c12, c22, c21;
+          walk<KMAX, SELF>(r1, z, sp, pr.x, pr.y, MA, c11, c12);
+          walk<KMAX, SELF>(r2, z, sp, pr.y, pr.x, MB, c22, c21);
+          const int w = s_me * task + j;
+          T.WA[w] = MA;
+          T.WB[w] = MB;
+          T.WC[w] = (uint32_t)c11 | ((uint32_t)c12 << 8) | ((uint32_t)c21 << 16) |
+                    ((uint32_t)c22 << 24);
+          x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, a, b, MA, MB, c11, c12, c21,
+                                        c22);
         }
-      }
-      if (q + 1 < nq) __syncthreads();
-    }
-    // argmin of (fitness bits, proposal index) over the shell-0 lanes;
-    // fitness >= 0, so its bit pattern orders like the value
-    const int nw0 = TP >> 6;
-    if (s_me == 0) {
-      unsigned long long rk = bkey;
-      int rj = bj;
-#pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) {
-        const unsigned long long ok =
-            ((unsigned long long)(uint32_t)__shfl_xor((int)(rk >> 32), m) << 32) |
-            (uint32_t)__shfl_xor((int)(uint32_t)rk, m);
-        const int oj = __shfl_xor(rj, m);
-        if (key_less(ok, oj, rk, rj)) {
-          rk = ok;
-          rj = oj;
-        }
-      }
-      if (lane == 0) {
-        T.sh->wkey[wave] = rk;
-        T.sh->wj[wave] = rj;
+        T.xs[s_me * task + j] = x;
       }
     }
     __syncthreads();
-    unsigned long long wk = T.sh->wkey[0];
-    int wj = T.sh->wj[0];
-    for (int w = 1; w < nw0; ++w)
-      if (key_less(T.sh->wkey[w], T.sh->wj[w], wk, wj)) {
-        wk = T.sh->wkey[w];
-        wj = T.sh->wj[w];
+    // ---- X = sum_s w_s x_s and its minimum
+    double lmin = __builtin_inf();
+    for (int j = tid; j < task; j += nthr) {
+      if (T.prop[j].x >= 0) {
+        double X = 0.0;
+        for (int s = 0; s < S; ++s) X = X + A.weight[s] * T.xs[s * task + j];
+        T.xs[j] = X;
+        lmin = fmin(lmin, X);
       }
-    const double best_fit = __longlong_as_double((long long)wk);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) lmin = fmin(lmin, __shfl_xor(lmin, m));
+    if (lane == 0) T.sh->wmin[wave] = lmin;
+    __syncthreads();
+
+    // ---- exact stage (wave 0): candidates within the window, first strict
+    // minimum of F (accelerate.cpp:663-674) and the accept test (:678)
+    if (wave == 0) {
+      double xmin = T.sh->wmin[0];
+#pragma unroll
+      for (int w = 1; w < kMaxWaves; ++w) xmin = fmin(xmin, T.sh->wmin[w]);
+      double thr = xmin + 2.0 * T.sh->beta;
+      if (!(thr < __builtin_inf())) thr = __builtin_inf();
+      int nc = 0;
+#pragma unroll 4
+      for (int j0 = 0; j0 < task; j0 += 64) {
+        const int j = j0 + lane;
+        const bool f = j < task && T.prop[j].x >= 0 && !(T.xs[j] > thr);
+        const unsigned long long bal = __ballot(f);
+        if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+        nc += __popcll(bal);
+      }
+      wave_sync();
+      // lane (c, s, x): candidate c of the chunk, shell s, species x
+      const int GS = S * K;      // lanes per candidate
+      const int cpc = 64 / GS;   // candidates per chunk
+      const int g = lane / K, x = lane - g * K;
+      const int c = g / S, s = g - c * S;
+      constexpr int XW = 4 + KMAX;
+      double* slot = T.xw + lane * XW;
+      const double* grp = T.xw + g * K * XW;  // lane (c, s, y) at grp + y * XW
+      unsigned long long bk = ~0ull;
+      int bj = 0x7fffffff, bc = 0, wc0 = -1;
+      for (int c0 = 0; c0 < nc; c0 += cpc) {
+        const int j = (lane < cpc * GS && c0 + c < nc) ? T.cl[c0 + c] : -1;
+        int a = 0, b = 0;
+        if (j >= 0) {
+          const int2 pr = T.prop[j];
+          a = sp[pr.x];
+          b = sp[pr.y];
+          const int w = s * task + j;
+          const uint32_t cp = T.WC[w];
+          double nA, nB;
+          exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w],
+                              (int)(cp & 255u), (int)((cp >> 8) & 255u),
+                              (int)((cp >> 16) & 255u), (int)(cp >> 24), nA, nB);
+          slot[0] = nA;
+          slot[1] = nB;
+          slot[2] = 1.0 - nA / (T.conc[a] * T.conc[x]);
+          slot[3] = 1.0 - nB / (T.conc[b] * T.conc[x]);
+        }
+        wave_sync();
+        if (j >= 0) {  // terms of row x (accelerate.cpp:560-566)
+          const double* tg = T.tgt + T.idx(s, x, 0);
+          const double* e0 = T.E0 + T.idx(s, x, 0);
+          const double sAx = slot[2], sBx = slot[3];
+#pragma unroll
+          for (int y = 0; y < KMAX; ++y) {
+            if (EXACT || y < K) {
+              const bool chg = (x == a) || (x == b) || (y == a) || (y == b);
+              const double sro = (x == a)   ? grp[y * XW + 2]
+                                 : (x == b) ? grp[y * XW + 3]
+                                 : (y == a) ? sAx
+                                            : sBx;
+              const double df = sro - tg[y];
+              slot[4 + y] = chg ? df * df : e0[y];
+            }
+          }
+        }
+        wave_sync();
+        if (j >= 0 && x == 0) {  // the shell's K*K terms in row-major order
+          double es = 0.0;
+#pragma unroll
+          for (int i = 0; i < KMAX; ++i)
+#pragma unroll
+            for (int y = 0; y < KMAX; ++y)
+              if (EXACT || (i < K && y < K)) es = es + grp[i * XW + 4 + y];
+          T.ex[g] = es;
+        }
+        wave_sync();
+        unsigned long long rk = ~0ull;
+        int rj = 0x7fffffff, rc = 0;
+        if (j >= 0 && s == 0 && x == 0) {
+          double err = 0.0;
+          for (int t = 0; t < S; ++t) err = err + A.weight[t] * T.ex[c * S + t];
+          rk = (unsigned long long)__double_as_longlong(sqrt(err));
+          rj = j;
+          rc = c;
+        }
+        if (nc - c0 == 1) {  // the usual case: one candidate, in lane 0
+          rk = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(rk >> 32)) << 32) |
+               (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)rk);
+          rj = __builtin_amdgcn_readfirstlane(rj);
+          rc = 0;
+        } else {
+#pragma unroll
+          for (int m = 32; m >= 1; m >>= 1) {
+            const unsigned long long ok =
+                ((unsigned long long)(uint32_t)__shfl_xor((int)(rk >> 32), m) << 32) |
+                (uint32_t)__shfl_xor((int)(uint32_t)rk, m);
+            const int oj = __shfl_xor(rj, m);
+            const int oc = __shfl_xor(rc, m);
+            if (key_less(ok, oj, rk, rj)) {
+              rk = ok;
+              rj = oj;
+              rc = oc;
+            }
+          }
+        }
+        if (key_less(rk, rj, bk, bj)) {
+          bk = rk;
+          bj = rj;
+          bc = rc;
+          wc0 = c0;
+        }
+        if (c0 + cpc < nc) wave_sync();  // the next chunk reuses xw and ex
+      }
+      const double best_fit = __longlong_as_double((long long)bk);
+      int a = 0, b = 0;
+      if (best_fit < prev_fit) {  // accept: write the winner's rows
+        const int2 pr = T.prop[bj];
+        a = sp[pr.x];
+        b = sp[pr.y];
+        if (wc0 + cpc < nc) {  // a later chunk overwrote the winner's entries
+          if (lane < GS) {
+            const int w = s * task + bj;
+            const uint32_t cp = T.WC[w];
+            double nA, nB;
+            exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w],
+                                (int)(cp & 255u), (int)((cp >> 8) & 255u),
+                                (int)((cp >> 16) & 255u), (int)(cp >> 24), nA, nB);
+            slot[0] = nA;
+            slot[1] = nB;
+          }
+          bc = 0;
+        }
+        wave_sync();
+        // rows a and b of every shell, with their mirrors (columns a and b)
+        for (int e = lane; e < S * 2 * K; e += 64) {
+          const int t = e / (2 * K), h = (e / K) & 1, y = e % K;
+          const double v = T.xw[((bc * S + t) * K + y) * XW + h];
+          const int r = h ? b : a;
+          T.G[T.idx(t, r, y)] = v;
+          T.G[T.idx(t, y, r)] = v;
+        }
+        if (lane < HEA_MAX_S) {  // the refresh below takes new maxima
+          T.sh->bmax[lane] = 0ull;
+          T.sh->ubmax[lane] = 0ull;
+        }
+      }
+      if (lane == 0) {
+        T.sh->res_key = bk;
+        T.sh->res_j = bj;
+        T.sh->res_a = a;
+        T.sh->res_b = b;
+      }
+    }
+    __syncthreads();
+    const double best_fit = __longlong_as_double((long long)T.sh->res_key);
     ++rounds;
 
     // ---- accept iff strictly better (accelerate.cpp:678-689)
     if (best_fit < prev_fit) {
-      const int2 pr = T.prop[wj];
-      const int a = sp[pr.x], b = sp[pr.y];
-      if (jl == wj % TP) {  // the winner's lane of every shell writes its rows
-        if (nq > 1) (void)eval_shell<KMAX, EXACT, SELF>(A, T, sp, s_me, pr.x, pr.y, nA, nB);
-#pragma unroll
-        for (int x = 0; x < KMAX; ++x) {
-          if (EXACT || x < K) {
-            T.G[T.idx(s_me, a, x)] = nA[x];
-            T.G[T.idx(s_me, x, a)] = nA[x];
-            T.G[T.idx(s_me, b, x)] = nB[x];
-            T.G[T.idx(s_me, x, b)] = nB[x];
-          }
-        }
-      }
+      const int2 pr = T.prop[T.sh->res_j];
+      const int a = T.sh->res_a, b = T.sh->res_b;
       prev_fit = best_fit;
       acc = 0;
       ++accepts;
       if (best_fit <= A.thr) hit_thr = true;
-      __syncthreads();
       if (tid == 0) {
         sp[pr.x] = (uint8_t)b;
         sp[pr.y] = (uint8_t)a;
       }
-      if (guided) {
+      if constexpr (GUIDED) {
         // neighbours of the swapped sites see a -> b (around pr.x) and
         // b -> a (around pr.y) in their histograms
         int zs = 0;
@@ -562,7 +814,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         refresh_entry(A, T, s, i, j);
       }
       __syncthreads();
-      if (guided) {
+      if constexpr (GUIDED) {
         // the histogram atomics went to L2: drop this CU's L1 copies
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         guide_tables<KMAX>(A, T);
@@ -589,17 +841,17 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   }
 }
 
-template <int KMAX, bool SP_LDS, bool EXACT, bool SELF>
+template <int KMAX, bool SP_LDS, bool EXACT, bool SELF, bool GUIDED>
 __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  mc_lattice<KMAX, SP_LDS, EXACT, SELF>(A, blockIdx.x, smem);
+  mc_lattice<KMAX, SP_LDS, EXACT, SELF, GUIDED>(A, blockIdx.x, smem);
 }
 
 // Persistent local search over many problems: every workgroup takes
 // (problem, lattice) items from a device queue, longest first, until the
 // queue is empty.  B.probs[p] holds problem p's complete launch arguments
 // (its seed is the batch seed + p).
-template <int KMAX, bool SP_LDS>
+template <int KMAX, bool SP_LDS, bool GUIDED>
 __global__ __launch_bounds__(1024) void k_local_mc_batch(HeaMcBatchArgs B) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int item_sh;
@@ -611,56 +863,88 @@ __global__ __launch_bounds__(1024) void k_local_mc_batch(HeaMcBatchArgs B) {
     if (item >= B.n_items) break;
     const int2 it = B.items[item];
     const HeaMcArgs& A = B.probs[it.x];
-    mc_lattice<KMAX, SP_LDS, false, false>(A, it.y, smem);
+    mc_lattice<KMAX, SP_LDS, false, false, GUIDED>(A, it.y, smem);
     __syncthreads();
   }
 }
 
-template <int KMAX, bool SP_LDS, bool EXACT, bool SELF>
+template <int KMAX, bool SP_LDS, bool EXACT, bool SELF, bool GUIDED>
 const void* mc_fn() {
-  return reinterpret_cast<const void*>(k_local_mc<KMAX, SP_LDS, EXACT, SELF>);
+  return reinterpret_cast<const void*>(k_local_mc<KMAX, SP_LDS, EXACT, SELF, GUIDED>);
+}
+
+template <int KMAX, bool EXACT, bool SELF, bool GUIDED>
+const void* mc_fn(bool sp_lds) {
+  return sp_lds ? mc_fn<KMAX, true, EXACT, SELF, GUIDED>() : mc_fn<KMAX, false, EXACT, SELF, GUIDED>();
 }
 
 // Kernel variant: exact-K instantiations for the common case, a guarded
 // K <= KMAX one for K = 7 and mixed batches, and a self-neighbour one for
-// cells so small that a site lists itself.
-const void* mc_fn_for(int K, bool sp_lds, bool exact_ok, bool self) {
-  if (self) return sp_lds ? mc_fn<8, true, false, true>() : mc_fn<8, false, false, true>();
+// cells so small that a site lists itself.  Guided proposals exist for
+// K <= 6 without self neighbours (the host checks this).
+const void* mc_fn_for(int K, bool sp_lds, bool exact_ok, bool self, bool guided) {
+  if (guided) {
+    if (self || K > 6) return nullptr;
+    if (exact_ok) {
+      switch (K) {
+        case 2: return mc_fn<2, true, false, true>(sp_lds);
+        case 3: return mc_fn<3, true, false, true>(sp_lds);
+        case 4: return mc_fn<4, true, false, true>(sp_lds);
+        case 5: return mc_fn<5, true, false, true>(sp_lds);
+        case 6: return mc_fn<6, true, false, true>(sp_lds);
+        default: break;
+      }
+    }
+    return mc_fn<6, false, false, true>(sp_lds);
+  }
+  if (self) return mc_fn<8, false, true, false>(sp_lds);
   if (exact_ok) {
     switch (K) {
-      case 2: return sp_lds ? mc_fn<2, true, true, false>() : mc_fn<2, false, true, false>();
-      case 3: return sp_lds ? mc_fn<3, true, true, false>() : mc_fn<3, false, true, false>();
-      case 4: return sp_lds ? mc_fn<4, true, true, false>() : mc_fn<4, false, true, false>();
-      case 5: return sp_lds ? mc_fn<5, true, true, false>() : mc_fn<5, false, true, false>();
-      case 6: return sp_lds ? mc_fn<6, true, true, false>() : mc_fn<6, false, true, false>();
-      case 8: return sp_lds ? mc_fn<8, true, true, false>() : mc_fn<8, false, true, false>();
+      case 2: return mc_fn<2, true, false, false>(sp_lds);
+      case 3: return mc_fn<3, true, false, false>(sp_lds);
+      case 4: return mc_fn<4, true, false, false>(sp_lds);
+      case 5: return mc_fn<5, true, false, false>(sp_lds);
+      case 6: return mc_fn<6, true, false, false>(sp_lds);
+      case 8: return mc_fn<8, true, false, false>(sp_lds);
       default: break;
     }
   }
-  if (K <= 6) return sp_lds ? mc_fn<6, true, false, false>() : mc_fn<6, false, false, false>();
-  if (K <= 8) return sp_lds ? mc_fn<8, true, false, false>() : mc_fn<8, false, false, false>();
+  if (K <= 6) return mc_fn<6, false, false, false>(sp_lds);
+  if (K <= 8) return mc_fn<8, false, false, false>(sp_lds);
   return nullptr;
 }
 
-const void* mc_batch_fn_for(int kmax, bool sp_lds) {
-  if (kmax <= 6)
-    return sp_lds ? reinterpret_cast<const void*>(k_local_mc_batch<6, true>)
-                  : reinterpret_cast<const void*>(k_local_mc_batch<6, false>);
-  return sp_lds ? reinterpret_cast<const void*>(k_local_mc_batch<8, true>)
-                : reinterpret_cast<const void*>(k_local_mc_batch<8, false>);
+template <int KMAX, bool GUIDED>
+const void* mc_batch_fn(bool sp_lds) {
+  return sp_lds ? reinterpret_cast<const void*>(k_local_mc_batch<KMAX, true, GUIDED>)
+                : reinterpret_cast<const void*>(k_local_mc_batch<KMAX, false, GUIDED>);
+}
+
+const void* mc_batch_fn_for(int kmax, bool sp_lds, bool guided) {
+  if (guided) return kmax <= 6 ? mc_batch_fn<6, true>(sp_lds) : nullptr;
+  return kmax <= 6 ? mc_batch_fn<6, false>(sp_lds) : mc_batch_fn<8, false>(sp_lds);
 }
 
 }  // namespace
 
+// Raises the batch kernels' dynamic-LDS limit (plain and, where it exists,
+// guided); call before launching or capturing.
 hipError_t hea_prepare_local_mc_batch(int kmax, bool sp_lds, size_t lds) {
-  return hipFuncSetAttribute(mc_batch_fn_for(kmax, sp_lds),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  for (int g = 0; g < 2; ++g) {
+    const void* fn = mc_batch_fn_for(kmax, sp_lds, g != 0);
+    if (!fn) continue;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
-hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_lds, int grid,
-                                     int threads, size_t lds, hipStream_t st) {
+hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_lds, bool guided,
+                                     int grid, int threads, size_t lds, hipStream_t st) {
+  const void* fn = mc_batch_fn_for(kmax, sp_lds, guided);
+  if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<HeaMcBatchArgs*>(&b)};
-  return hipLaunchKernel(mc_batch_fn_for(kmax, sp_lds), dim3(grid), dim3(threads), args, lds, st);
+  return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
 }
 
 // Table width of the kernel chosen for K species (see mc_fn_for).
@@ -677,28 +961,34 @@ int hea_mc_tp(int task, int S) {
   return std::min(want, cap);
 }
 
+// LDS bytes of one workgroup (the pass width does not change the layout).
 size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds) {
+  (void)tp;
   switch (kmax) {
-    case 2: return Lds<2>::bytes(task, tp, n_sites, sp_lds);
-    case 3: return Lds<3>::bytes(task, tp, n_sites, sp_lds);
-    case 4: return Lds<4>::bytes(task, tp, n_sites, sp_lds);
-    case 5: return Lds<5>::bytes(task, tp, n_sites, sp_lds);
-    case 6: return Lds<6>::bytes(task, tp, n_sites, sp_lds);
-    default: return Lds<8>::bytes(task, tp, n_sites, sp_lds);
+    case 2: return Lds<2>::bytes(task, n_sites, sp_lds);
+    case 3: return Lds<3>::bytes(task, n_sites, sp_lds);
+    case 4: return Lds<4>::bytes(task, n_sites, sp_lds);
+    case 5: return Lds<5>::bytes(task, n_sites, sp_lds);
+    case 6: return Lds<6>::bytes(task, n_sites, sp_lds);
+    default: return Lds<8>::bytes(task, n_sites, sp_lds);
   }
 }
 
-// Raises the kernel's dynamic-LDS limit; call before launching (and before
-// any graph capture that contains the launch).
+// Raises the kernel's dynamic-LDS limit (plain and guided variants); call
+// before launching (and before any graph capture that contains the launch).
 hipError_t hea_prepare_local_mc(int K, bool sp_lds, bool self, size_t lds) {
-  const void* fn = mc_fn_for(K, sp_lds, true, self);
+  const void* fn = mc_fn_for(K, sp_lds, true, self, false);
   if (!fn) return hipErrorInvalidValue;
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void* gfn = mc_fn_for(K, sp_lds, true, self, true);
+  if (e == hipSuccess && gfn)
+    e = hipFuncSetAttribute(gfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return e;
 }
 
 hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self, size_t lds,
                                hipStream_t st) {
-  const void* fn = mc_fn_for(K, sp_lds, true, self);
+  const void* fn = mc_fn_for(K, sp_lds, true, self, a.hist != nullptr);
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<HeaMcArgs*>(&a)};
   return hipLaunchKernel(fn, dim3(a.L), dim3(a.S * a.tp), args, lds, st);
