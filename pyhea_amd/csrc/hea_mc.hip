@@ -71,7 +71,7 @@ struct McShared {
   unsigned long long res_key;           // exact stage: winner's fitness bits
   int res_j, res_a, res_b;
   uint32_t next_pos;
-  unsigned gmax;  // guided mode: float bits of max_s Z_s |D| (gain scale)
+  unsigned gmax;  // guided mode: float bits of max_s Z_s |H_s| (gain scale)
 };
 
 // Species histogram of one shell walk, packed one field per species:
@@ -132,27 +132,83 @@ __device__ __forceinline__ double replay_ab(double v, double delta, const int32_
   return v;
 }
 
-// Species histogram of one neighbour list: M += 1<<(W*species); cs counts
-// `self`, co counts `other`.  Rows are padded to a multiple of 4 entries.
-template <int KMAX, bool SELF>
-__device__ __forceinline__ void walk(const int32_t* __restrict__ row, int z,
-                                     const uint8_t* __restrict__ sp, int self, int other,
-                                     typename Pack<KMAX>::T& M, int& cs, int& co) {
-  M = 0;
-  cs = 0;
-  co = 0;
-  for (int q = 0; q < z; q += 4) {
-    const int4 v = *reinterpret_cast<const int4*>(row + q);
-    const int nb[4] = {v.x, v.y, v.z, v.w};
+// Species histograms of the shell-s neighbour lists of a1 and a2 (MA, MB:
+// += 1 << (W * species)) and the adjacency counts cXY (occurrences of site Y
+// in X's list).  Lists are padded to a multiple of 4 entries with -1.  NC =
+// ceil(z/4) 16-byte chunks per list; chunks C0..C0+NG-1 of both lists are
+// loaded before any is used, so their loads are in flight together.
+template <int KMAX, bool SELF, int C0, int NG, bool LAST>
+__device__ __forceinline__ void walk_group(const int32_t* __restrict__ r1,
+                                           const int32_t* __restrict__ r2, int z,
+                                           const uint8_t* __restrict__ sp, int a1, int a2,
+                                           typename Pack<KMAX>::T& ma, typename Pack<KMAX>::T& mb,
+                                           int& x11, int& x12, int& x21, int& x22) {
+  using PK = Pack<KMAX>;
+  int4 u[NG], v[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    u[c] = reinterpret_cast<const int4*>(r1)[C0 + c];
+    v[c] = reinterpret_cast<const int4*>(r2)[C0 + c];
+  }
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    const int nu[4] = {u[c].x, u[c].y, u[c].z, u[c].w};
+    const int nv[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      if (q + t < z) {
-        const int n = nb[t];
-        M += Pack<KMAX>::one((int)sp[n]);
-        if (SELF) cs += (n == self);
-        co += (n == other);
+      const bool ok = !(LAST && c == NG - 1) || (4 * (C0 + c) + t < z);  // uniform
+      const typename PK::T fa = PK::one((int)sp[ok ? nu[t] : a1]);
+      const typename PK::T fb = PK::one((int)sp[ok ? nv[t] : a2]);
+      ma += ok ? fa : (typename PK::T)0;
+      mb += ok ? fb : (typename PK::T)0;
+      x12 += (nu[t] == a2);  // padding (-1) never matches
+      x21 += (nv[t] == a1);
+      if (SELF) {
+        x11 += (nu[t] == a1);
+        x22 += (nv[t] == a2);
       }
     }
+    // one chunk's LDS reads at a time (the group's row loads are in flight)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Both lists of one shell in groups of at most 4 chunks (registers bound the
+// loads in flight).
+template <int KMAX, bool SELF, int NC>
+__device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
+                                          const int32_t* __restrict__ r2, int z,
+                                          const uint8_t* __restrict__ sp, int a1, int a2,
+                                          typename Pack<KMAX>::T& MA, typename Pack<KMAX>::T& MB,
+                                          int& c11, int& c12, int& c21, int& c22) {
+  typename Pack<KMAX>::T ma = 0, mb = 0;
+  int x11 = 0, x12 = 0, x21 = 0, x22 = 0;
+  if constexpr (NC <= 4) {
+    walk_group<KMAX, SELF, 0, NC, true>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
+  } else {
+    walk_group<KMAX, SELF, 0, 4, false>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
+    walk_group<KMAX, SELF, 4, NC - 4, true>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
+  }
+  MA = ma;
+  MB = mb;
+  c11 = x11;
+  c12 = x12;
+  c21 = x21;
+  c22 = x22;
+}
+
+template <int KMAX, bool SELF>
+__device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
+                                          const int32_t* __restrict__ r2, int z,
+                                          const uint8_t* __restrict__ sp, int a1, int a2,
+                                          typename Pack<KMAX>::T& MA, typename Pack<KMAX>::T& MB,
+                                          int& c11, int& c12, int& c21, int& c22) {
+  switch ((z + 3) >> 2) {  // uniform across the wave (one shell per wave)
+#define HEA_WALK(n) \
+  case n: walk_pair<KMAX, SELF, n>(r1, r2, z, sp, a1, a2, MA, MB, c11, c12, c21, c22); break;
+    HEA_WALK(1) HEA_WALK(2) HEA_WALK(3) HEA_WALK(4) HEA_WALK(5) HEA_WALK(6) HEA_WALK(7) HEA_WALK(8)
+#undef HEA_WALK
+    default: break;
   }
 }
 
@@ -161,7 +217,7 @@ template <int KMAX>
 struct LdsLayout {
   static constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
   using PT = typename Pack<KMAX>::T;
-  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, WA, WB, WC, cl, sh, sp, total;
+  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, gkey, WA, WB, WC, cl, sh, sp, total;
   __host__ __device__ LdsLayout(int task, int n, bool sp_lds) {
     size_t o = 0;
     auto take = [&o](size_t bytes) {
@@ -174,12 +230,13 @@ struct LdsLayout {
     R = take(sizeof(double) * TAB);
     tgt = take(sizeof(double) * TAB);
     SC = take(sizeof(double2) * TAB);
-    H = take(sizeof(double) * TAB + sizeof(float) * HEA_MAX_S * KMAX * KMAX * KMAX);
+    H = take(sizeof(float) * TAB);
     conc = take(sizeof(double) * KMAX);
     xs = take(sizeof(double) * HEA_MAX_S * (size_t)task);
     ex = take(sizeof(double) * 64);
     xw = take(sizeof(double) * 64 * (4 + KMAX));
     prop = take(sizeof(int2) * (size_t)task);
+    gkey = take(sizeof(unsigned long long) * (size_t)task);
     WA = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
     WB = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
     WC = take(sizeof(uint32_t) * HEA_MAX_S * (size_t)task);
@@ -199,13 +256,14 @@ struct Lds {
   double* R;     // [S][KMAX][KMAX] safe step or 0
   double* tgt;   // [S][KMAX][KMAX] target alpha
   double2* SC;   // [S][KMAX][KMAX] {d_ij + d_ji, -delta_s/dd_ij}, d = alpha - t
-  double* H;     // [S][KMAX][KMAX] guided mode: dF2/dn + its transpose; then D floats
+  float* H;      // [S][KMAX][KMAX] guided mode: dF2/dn + its transpose
   double* conc;  // [KMAX] fl(count/N)
   double* xs;    // [S][task] screen values; row 0 then holds the combined X
   double* ex;    // [64] exact-stage shell sums (wave 0)
   double* xw;    // [64][4 + KMAX] exact stage, lane (c, s, x): gamma'(a,x), gamma'(b,x),
                  // their alphas, then the terms of row x
   int2* prop;    // [task]
+  unsigned long long* gkey;  // [task] guided mode: best (score, candidate) of each slot
   PT* WA;        // [S][task] walk results: histogram of a1's neighbours
   PT* WB;        // [S][task] histogram of a2's neighbours
   uint32_t* WC;  // [S][task] c11 | c12 << 8 | c21 << 16 | c22 << 24
@@ -220,12 +278,13 @@ struct Lds {
     R = reinterpret_cast<double*>(base + L.R);
     tgt = reinterpret_cast<double*>(base + L.tgt);
     SC = reinterpret_cast<double2*>(base + L.SC);
-    H = reinterpret_cast<double*>(base + L.H);
+    H = reinterpret_cast<float*>(base + L.H);
     conc = reinterpret_cast<double*>(base + L.conc);
     xs = reinterpret_cast<double*>(base + L.xs);
     ex = reinterpret_cast<double*>(base + L.ex);
     xw = reinterpret_cast<double*>(base + L.xw);
     prop = reinterpret_cast<int2*>(base + L.prop);
+    gkey = reinterpret_cast<unsigned long long*>(base + L.gkey);
     WA = reinterpret_cast<PT*>(base + L.WA);
     WB = reinterpret_cast<PT*>(base + L.WB);
     WC = reinterpret_cast<uint32_t*>(base + L.WC);
@@ -268,7 +327,7 @@ __device__ __forceinline__ void wave_sync() {
 // beta = beta0 + 4 u E_up, any proposal with X > Xmin + 2 beta has
 // err > err(Pmin)(1 + 8u), hence sqrt(err) rounds strictly above the
 // minimum's F and cannot be the reference's choice.
-template <int KMAX>
+template <int KMAX, bool GUIDED>
 __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int i, int j) {
   const int e = T.idx(s, i, j);
   const double g = T.G[e];
@@ -295,6 +354,14 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
   const double ub = D1 + Ae + e1 + e2;
   atomicMax(&T.sh->bmax[s], (unsigned long long)__double_as_longlong(P * 1.0001));
   atomicMax(&T.sh->ubmax[s], (unsigned long long)__double_as_longlong(ub * 1.0001));
+  if constexpr (GUIDED) {
+    // H_ij = dF2/dn_ij + dF2/dn_ji of the fitness squared
+    // (accelerate.cpp:556-569) with gamma = n delta: dF2/dn_ij =
+    // -2 w (alpha_ij - t_ij) delta / dd_ij = 2 w d_ij c_ij, c_ij = -delta/dd_ij
+    const float h = (float)(2.0 * A.weight[s] * (dij + dji) * T.SC[e].y);
+    T.H[e] = h;
+    atomicMax(&T.sh->gmax, __float_as_uint(fabsf(h) * (float)A.Z[s]));
+  }
 }
 
 // beta of the refreshed tables (see refresh_entry).
@@ -392,37 +459,18 @@ __device__ __forceinline__ void exact_entries(const HeaMcArgs& A, const Lds<KMAX
   }
 }
 
-// Guided mode: H_s = dF2/dn_s + (dF2/dn_s)^T with the closed-form
-// dF2/dn_sab = -2 w_s (alpha_sab - t_sab) / (Z_s N c_a c_b) of the reference
-// fitness squared (accelerate.cpp:556-569) at gamma = n / (Z_s N); then
-// D[s][a][b][d] = H_s[b][d] - H_s[a][d], so that swapping i (species a) with
-// j (species b) changes F^2 by sum_s sum_d D[s][a][b][d] (cnt_s[i][d] -
-// cnt_s[j][d]) to first order (cnt: neighbour species counts).
-template <int KMAX>
-__device__ void guide_tables(const HeaMcArgs& A, const Lds<KMAX>& T) {
-  const int K = A.K, S = A.S, N = A.N, nthr = blockDim.x, tid = threadIdx.x;
-  for (int e = tid; e < S * K * K; e += nthr) {
-    const int s = e / (K * K), c = (e / K) % K, b = e % K;
-    const double zn = (double)A.Z[s] * N;
-    const double ccb = T.conc[c] * T.conc[b];
-    const double dcb =
-        -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, c, b)] / ccb) - T.tgt[T.idx(s, c, b)]) / (zn * ccb);
-    const double dbc =
-        -2.0 * A.weight[s] * ((1.0 - T.G[T.idx(s, b, c)] / ccb) - T.tgt[T.idx(s, b, c)]) / (zn * ccb);
-    T.H[T.idx(s, c, b)] = dcb + dbc;
-  }
-  if (tid == 0) T.sh->gmax = 0u;
-  __syncthreads();
-  float* D = reinterpret_cast<float*>(T.H + Lds<KMAX>::TAB);
-  unsigned mx = 0u;
-  for (int e = tid; e < S * K * K * K; e += nthr) {
-    const int s = e / (K * K * K), a = (e / (K * K)) % K, b = (e / K) % K, d = e % K;
-    const float v = (float)(T.H[T.idx(s, b, d)] - T.H[T.idx(s, a, d)]);
-    D[((s * KMAX + a) * KMAX + b) * KMAX + d] = v;
-    mx = max(mx, __float_as_uint(fabsf(v) * (float)A.Z[s]));
-  }
-  atomicMax(&T.sh->gmax, mx);
-  __syncthreads();
+// Guided mode: candidate item `ctr` of a lattice's candidate stream (key gk):
+// two sites, drawn by multiply-high from independent hashes, and a 32-bit
+// word for the Gumbel noise.  This stream feeds only the guided proposal
+// choice; the exact acceptance never depends on it.
+__device__ __forceinline__ void guided_pair(uint32_t gk, uint32_t ctr, uint32_t n, uint32_t* s1,
+                                            uint32_t* s2, uint32_t* h3) {
+  const uint32_t h0 = hea_mix32(gk ^ (ctr * 0x9e3779b9u));
+  const uint32_t h1 = hea_mix32(h0 ^ 0x85ebca6bu);
+  const uint32_t h2 = hea_mix32(h1 ^ 0xc2b2ae35u);
+  *s1 = __umulhi(h0, n);
+  *s2 = __umulhi(h1, n);
+  *h3 = h2;
 }
 
 __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned long long f2,
@@ -469,6 +517,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     T.sh->bmax[tid] = 0ull;
     T.sh->ubmax[tid] = 0ull;
   }
+  if (tid == 0) T.sh->gmax = 0u;
+  if constexpr (GUIDED)
+    for (int j = tid; j < task; j += nthr) T.gkey[j] = 0ull;
   if (tid < kMaxWaves) {  // slots of absent waves stay neutral
     T.sh->wcount[tid] = 0;
     T.sh->wmin[tid] = __builtin_inf();
@@ -476,7 +527,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   __syncthreads();
   for (int e = tid; e < S * K * K; e += nthr) {
     const int s = e / (K * K), i = (e / K) % K, j = e % K;
-    refresh_entry(A, T, s, i, j);
+    refresh_entry<KMAX, GUIDED>(A, T, s, i, j);
   }
   __syncthreads();
 
@@ -499,63 +550,93 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         for (int q = 0; q < A.Z[s]; ++q) m[s] += 1u << (5 * sp[row[A.off[s] + q]]);
       *reinterpret_cast<uint4*>(hist + (size_t)k * 4) = make_uint4(m[0], m[1], m[2], m[3]);
     }
-    guide_tables<KMAX>(A, T);
+    __syncthreads();  // every site's histogram before the first candidate reads
   }
+  // guided mode: candidate stream key of this lattice and phase
+  const uint32_t gk = hea_mix32(hea_mix32(hea_mix32((uint32_t)A.seed ^ 0x6a09e667u) ^
+                                          (uint32_t)(A.seed >> 32)) ^
+                                (iter * 4u + (A.phase & 3u))) ^
+                      hea_mix32((uint32_t)l + 0x3c6ef372u);
 
   while (acc < A.depth) {
     // window of this round's tables; published by the barriers below
     if (tid == 0) T.sh->beta = screen_beta<KMAX>(A, T);
     if constexpr (GUIDED) {
-      // ---- guided: each slot takes the Gumbel-max of -gain/(tau*scale)
-      // over g_cand keyed candidate pairs (same-species candidates skipped)
-      const float* D = reinterpret_cast<const float*>(T.H + Lds<KMAX>::TAB);
-      const float gs = __uint_as_float(T.sh->gmax);
+      // ---- guided: slot j takes the Gumbel-max of -gain/(tau*scale) over
+      // g_cand hashed candidate pairs (same-species pairs skipped); all lanes
+      // score (slot, candidate) items and keep each slot's best in gkey
+      const float gs = 2.0f * __uint_as_float(T.sh->gmax);
       const float inv = (float)(1.0 / (A.g_tau * (gs > 0.0f ? (double)gs : 1.0)));
-      const int c4 = (A.g_cand + 3) & ~3;  // candidates per slot, in groups of 4
-      for (int j = tid; j < task; j += nthr) {
-        float best = -3.0e38f;
-        int2 bp = make_int2(-1, -1);
-        for (int c0 = 0; c0 < A.g_cand; c0 += 4) {
-          const uint32_t pi0 = pos + (uint32_t)(j * c4 + c0);  // multiple of 4
-          const hea_u32x4 w01 = hea_stream_block(A.seed, iter, A.phase, (uint32_t)l, pi0 >> 1);
-          const hea_u32x4 w23 = hea_stream_block(A.seed, iter, A.phase, (uint32_t)l, (pi0 >> 1) + 1);
-          const hea_u32x4 nz = hea_stream_block(A.seed, iter, A.phase | 0x100u, (uint32_t)l, pi0 >> 2);
-          const uint32_t ws[8] = {w01.x, w01.y, w01.z, w01.w, w23.x, w23.y, w23.z, w23.w};
-          const uint32_t ns[4] = {nz.x, nz.y, nz.z, nz.w};
+      const int g = A.g_cand, items = task * g;
+      // four items per lane at a time, so their histogram loads overlap
+      for (int w0 = tid; w0 < items; w0 += 4 * nthr) {
+        uint32_t s1[4], s2[4], h3[4];
+        int sa[4], sb[4];
+        bool live[4];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            if (c0 + c < A.g_cand) {
-              const uint32_t s1 = ws[2 * c] % (uint32_t)N, s2 = ws[2 * c + 1] % (uint32_t)N;
-              const int a = sp[s1], b = sp[s2];
-              if (a != b) {
-                const uint4 h1 = *reinterpret_cast<const uint4*>(hist + (size_t)s1 * 4);
-                const uint4 h2 = *reinterpret_cast<const uint4*>(hist + (size_t)s2 * 4);
-                const uint32_t hh1[3] = {h1.x, h1.y, h1.z}, hh2[3] = {h2.x, h2.y, h2.z};
-                float d = 0.0f;
+        for (int q = 0; q < 4; ++q) {
+          const int w = w0 + q * nthr;
+          guided_pair(gk, pos + (uint32_t)w, (uint32_t)N, &s1[q], &s2[q], &h3[q]);
+          sa[q] = sp[s1[q]];
+          sb[q] = sp[s2[q]];
+          live[q] = w < items && sa[q] != sb[q];
+        }
+        unsigned long long hv[4][4];
 #pragma unroll
-                for (int s = 0; s < HEA_MAX_S; ++s) {
-                  if (s < S) {
-                    const float* Dr = D + ((s * KMAX + a) * KMAX + b) * KMAX;
+        for (int q = 0; q < 4; ++q) {
+          // every access to a lattice's histograms comes from its own
+          // workgroup: workgroup-scope atomics, loads and barriers order them
+          const unsigned long long* p1 = reinterpret_cast<const unsigned long long*>(hist + (size_t)s1[q] * 4);
+          const unsigned long long* p2 = reinterpret_cast<const unsigned long long*>(hist + (size_t)s2[q] * 4);
+          hv[q][0] = __hip_atomic_load(p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          hv[q][1] = __hip_atomic_load(p1 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          hv[q][2] = __hip_atomic_load(p2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          hv[q][3] = __hip_atomic_load(p2 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
 #pragma unroll
-                    for (int q = 0; q < KMAX; ++q)
-                      if (q < K)
-                        d += Dr[q] * (float)((int)((hh1[s] >> (5 * q)) & 31u) -
-                                             (int)((hh2[s] >> (5 * q)) & 31u));
-                  }
-                }
-                const float U = ((float)ns[c] + 0.5f) * (1.0f / 2147483648.0f);
-                const float sc = -d * inv - __logf(-__logf(U));
-                if (sc > best) {
-                  best = sc;
-                  bp = make_int2((int)s1, (int)s2);
-                }
-              }
+        for (int q = 0; q < 4; ++q) {
+          if (!live[q]) continue;
+          const int a = sa[q], b = sb[q];
+          const uint32_t hh1[3] = {(uint32_t)hv[q][0], (uint32_t)(hv[q][0] >> 32), (uint32_t)hv[q][1]};
+          const uint32_t hh2[3] = {(uint32_t)hv[q][2], (uint32_t)(hv[q][2] >> 32), (uint32_t)hv[q][3]};
+          // first-order change of F^2: sum_s sum_d (H_s[b][d] - H_s[a][d]) (cnt1 - cnt2)
+          float d = 0.0f;
+#pragma unroll
+          for (int sh = 0; sh < HEA_MAX_S; ++sh) {
+            if (sh < S) {
+              const float* Ha = T.H + T.idx(sh, a, 0);
+              const float* Hb = T.H + T.idx(sh, b, 0);
+#pragma unroll
+              for (int x = 0; x < KMAX; ++x)
+                if (EXACT || x < K)
+                  d += (Hb[x] - Ha[x]) * (float)((int)((hh1[sh] >> (5 * x)) & 31u) -
+                                                 (int)((hh2[sh] >> (5 * x)) & 31u));
             }
           }
+          const float U = ((float)(h3[q] >> 8) + 0.5f) * 0x1p-24f;
+          const float sc = -d * inv - __logf(-__logf(U));
+          // order-preserving float bits; ties go to the lower candidate
+          const uint32_t fb = __float_as_uint(sc);
+          const uint32_t ob = (fb & 0x80000000u) ? ~fb : (fb | 0x80000000u);
+          const int w = w0 + q * nthr;
+          const int j = w / g, c = w - j * g;
+          atomicMax(&T.gkey[j], ((unsigned long long)ob << 32) | (uint32_t)(0xffffffffu - (uint32_t)c));
+        }
+      }
+      __syncthreads();
+      for (int j = tid; j < task; j += nthr) {
+        const unsigned long long key = T.gkey[j];
+        T.gkey[j] = 0ull;
+        int2 bp = make_int2(-1, -1);
+        if (key != 0ull) {
+          const int c = (int)(0xffffffffu - (uint32_t)key);
+          uint32_t s1, s2, h3;
+          guided_pair(gk, pos + (uint32_t)(j * g + c), (uint32_t)N, &s1, &s2, &h3);
+          bp = make_int2((int)s1, (int)s2);
         }
         T.prop[j] = bp;
       }
-      pos += (uint32_t)(task * ((A.g_cand + 3) & ~3));
+      pos += (uint32_t)items;
       __syncthreads();
     }
     // ---- draw `task` valid proposals in stream order (accelerate.cpp:649-655)
@@ -596,10 +677,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           const int32_t* r1 = A.nbr + (size_t)pr.x * A.stride + A.off[s_me];
           const int32_t* r2 = A.nbr + (size_t)pr.y * A.stride + A.off[s_me];
           const int z = A.Z[s_me];
-          PT MA, MB;
-          int c11, c12, c22, c21;
-          walk<KMAX, SELF>(r1, z, sp, pr.x, pr.y, MA, c11, c12);
-          walk<KMAX, SELF>(r2, z, sp, pr.y, pr.x, MB, c22, c21);
+          PT MA = 0, MB = 0;
+          int c11 = 0, c12 = 0, c22 = 0, c21 = 0;
+          walk_pair<KMAX, SELF>(r1, r2, z, sp, pr.x, pr.y, MA, MB, c11, c12, c21, c22);
           const int w = s_me * task + j;
           T.WA[w] = MA;
           T.WB[w] = MB;
@@ -635,13 +715,18 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       for (int w = 1; w < kMaxWaves; ++w) xmin = fmin(xmin, T.sh->wmin[w]);
       double thr = xmin + 2.0 * T.sh->beta;
       if (!(thr < __builtin_inf())) thr = __builtin_inf();
+      const int nk = (task + 63) >> 6;  // <= 32
+      uint32_t fl = 0u;                  // bit k: slot k*64 + lane is a candidate
+#pragma unroll 8
+      for (int k = 0; k < nk; ++k) {
+        const int j = k * 64 + lane;
+        if (j < task && T.prop[j].x >= 0 && !(T.xs[j] > thr)) fl |= 1u << k;
+      }
       int nc = 0;
-#pragma unroll 4
-      for (int j0 = 0; j0 < task; j0 += 64) {
-        const int j = j0 + lane;
-        const bool f = j < task && T.prop[j].x >= 0 && !(T.xs[j] > thr);
+      for (int k = 0; k < nk; ++k) {
+        const bool f = (fl >> k) & 1u;
         const unsigned long long bal = __ballot(f);
-        if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+        if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = k * 64 + lane;
         nc += __popcll(bal);
       }
       wave_sync();
@@ -771,6 +856,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           T.sh->bmax[lane] = 0ull;
           T.sh->ubmax[lane] = 0ull;
         }
+        if (lane == 0) T.sh->gmax = 0u;
       }
       if (lane == 0) {
         T.sh->res_key = bk;
@@ -806,19 +892,15 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           while (q >= A.Z[s]) q -= A.Z[s++];
           const int nb = A.nbr[(size_t)site * A.stride + A.off[s] + q];
           const uint32_t dv = t < zs ? (1u << (5 * b)) - (1u << (5 * a)) : (1u << (5 * a)) - (1u << (5 * b));
-          atomicAdd(hist + (size_t)nb * 4 + s, dv);
+          __hip_atomic_fetch_add(hist + (size_t)nb * 4 + s, dv, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
       for (int e = tid; e < S * K * K; e += nthr) {
         const int s = e / (K * K), i = (e / K) % K, j = e % K;
-        refresh_entry(A, T, s, i, j);
+        refresh_entry<KMAX, GUIDED>(A, T, s, i, j);
       }
       __syncthreads();
-      if constexpr (GUIDED) {
-        // the histogram atomics went to L2: drop this CU's L1 copies
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        guide_tables<KMAX>(A, T);
-      }
       if (hit_thr && tid == 0 && A.first_hit) {
         atomicMin(A.first_hit, (unsigned long long)wall_clock64());
         hit_thr = false;
