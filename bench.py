@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--budget-s", type=float, default=60.0,
                     help="time-to-fitness workloads: wall-clock budget per mode")
     ap.add_argument("--thr", type=float, default=1e-3)
+    ap.add_argument("--guide-candidates", type=int, default=8,
+                    help="time-to-fitness workloads: candidate pairs per guided proposal")
+    ap.add_argument("--guide-accepts", type=int, default=8,
+                    help="time-to-fitness workloads: guided accepts per round at most")
     ap.add_argument("--dump-problem", default=None,
                     help="write the workload's problem for pyhea_amd/hea_profile and exit")
     ap.add_argument("--profile-steps", type=int, default=3,
@@ -141,7 +145,7 @@ def time_to_fitness(a):
     for mode in ("plain", "guided"):
         s = prob.session(L, T, D, thr=a.thr, seed=seed)
         if mode == "guided":
-            s.guide(candidates=8, tau=0.1)
+            s.guide(candidates=a.guide_candidates, tau=0.1, accepts=a.guide_accepts)
         t0 = time.perf_counter()
         s.init()
         trace = []

@@ -143,10 +143,14 @@ int hea_session_tile(hea_session s, int32_t tile);
  * current lattice) up to date; the predicted first-order change of swapping
  * i (species a) with j (species b) is u[i][b] + u[j][a].  Each of the
  * parallel_task proposals of a round is the Gumbel-max of
- * -(u[i][b] + u[j][a]) / (tau * scale) over `candidates` keyed random pairs.
- * Acceptance stays the exact rule of accelerate.cpp:663-689.  Needs K <= 6
- * and no self neighbours.  Call before hea_session_init/step. */
-int hea_session_guide(hea_session s, int32_t candidates, double tau);
+ * -(u[i][b] + u[j][a]) / (tau * scale) over `candidates` hashed random pairs.
+ * The round's best proposal is accepted by the exact rule of
+ * accelerate.cpp:663-689; then up to accepts - 1 more come from the same
+ * round: the proposals that touch none of the swapped sites or their
+ * neighbours are screened again against the new lattice, and the best is
+ * accepted iff its exact fitness is lower.  Needs K <= 6 and no self
+ * neighbours.  Call before hea_session_init/step. */
+int hea_session_guide(hea_session s, int32_t candidates, double tau, int32_t accepts);
 /* Runs `n_iter` global iterations asynchronously on the session stream. */
 int hea_session_step(hea_session s, int32_t n_iter);
 /* Measurement helper with no counterpart in the reference: runs n_iter
@@ -180,7 +184,7 @@ int hea_batch_grid(hea_batch b, int32_t grid);
 /* Proposals per pass of the batch's persistent launch (see hea_session_tile). */
 int hea_batch_tile(hea_batch b, int32_t tile);
 /* Guided proposals for every problem of the batch (see hea_session_guide). */
-int hea_batch_guide(hea_batch b, int32_t candidates, double tau);
+int hea_batch_guide(hea_batch b, int32_t candidates, double tau, int32_t accepts);
 int hea_batch_init(hea_batch b);
 int hea_batch_step(hea_batch b, int32_t n_iter);
 int hea_batch_sync(hea_batch b);

@@ -110,7 +110,7 @@ def _declare(lib: C.CDLL) -> None:
         "hea_session_create": (C.c_int, [P, I32, I32, I32, D, U64, C.POINTER(P)]),
         "hea_session_init": (C.c_int, [P]),
         "hea_session_tile": (C.c_int, [P, I32]),
-        "hea_session_guide": (C.c_int, [P, I32, D]),
+        "hea_session_guide": (C.c_int, [P, I32, D, I32]),
         "hea_session_step": (C.c_int, [P, I32]),
         "hea_session_step_timed": (C.c_int, [P, I32, C.POINTER(C.c_float),
                                              C.POINTER(C.c_int64)]),
@@ -121,7 +121,7 @@ def _declare(lib: C.CDLL) -> None:
         "hea_batch_create": (C.c_int, [C.POINTER(P), I32, I32, I32, I32, D, U64, C.POINTER(P)]),
         "hea_batch_grid": (C.c_int, [P, I32]),
         "hea_batch_tile": (C.c_int, [P, I32]),
-        "hea_batch_guide": (C.c_int, [P, I32, D]),
+        "hea_batch_guide": (C.c_int, [P, I32, D, I32]),
         "hea_batch_init": (C.c_int, [P]),
         "hea_batch_step": (C.c_int, [P, I32]),
         "hea_batch_sync": (C.c_int, [P]),

@@ -185,6 +185,7 @@ struct GuideCfg {
   uint32_t* hist = nullptr;  // [L][N][S] packed neighbour species counts
   int cand = 0;
   double tau = 0.0;
+  int multi = 1;
 };
 
 }  // namespace
@@ -374,6 +375,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.hist = gd.hist;
   a.g_cand = gd.cand;
   a.g_tau = gd.tau;
+  a.g_multi = gd.multi;
   const HeaConst& c = p->hc;
   a.N = c.N;
   a.K = c.K;
@@ -792,16 +794,17 @@ int hea_session_tile(hea_session s, int32_t tp) {
   return HEA_OK;
 }
 
-static int check_guide(hea_problem p, int candidates, double tau) {
-  if (candidates < 1 || !(tau > 0.0)) return fail(HEA_EINVAL, "guide needs candidates >= 1, tau > 0");
+static int check_guide(hea_problem p, int candidates, double tau, int accepts) {
+  if (candidates < 1 || !(tau > 0.0) || accepts < 1)
+    return fail(HEA_EINVAL, "guide needs candidates >= 1, tau > 0, accepts >= 1");
   if (p->hc.K > 6) return fail(HEA_EUNSUPPORTED, "guided proposals support up to 6 species");
   if (p->hc.self_nbr) return fail(HEA_EUNSUPPORTED, "guided proposals need cells without self neighbours");
   return HEA_OK;
 }
 
-int hea_session_guide(hea_session s, int32_t candidates, double tau) {
+int hea_session_guide(hea_session s, int32_t candidates, double tau, int32_t accepts) {
   if (!s) return fail(HEA_EINVAL, "null session");
-  int rc = check_guide(s->p, candidates, tau);
+  int rc = check_guide(s->p, candidates, tau, accepts);
   if (rc) return rc;
   HEA_HIP(hipSetDevice(s->p->device));
   const size_t n = (size_t)s->L * s->p->hc.N * 4;
@@ -814,6 +817,7 @@ int hea_session_guide(hea_session s, int32_t candidates, double tau) {
   s->guide.hist = s->hist.p;
   s->guide.cand = candidates;
   s->guide.tau = tau;
+  s->guide.multi = accepts;
   if (s->exec) {  // the graph bakes the launch arguments in: capture again
     (void)hipGraphExecDestroy(s->exec);
     (void)hipGraphDestroy(s->graph);
@@ -1204,11 +1208,11 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   return HEA_OK;
 }
 
-int hea_batch_guide(hea_batch b, int32_t candidates, double tau) {
+int hea_batch_guide(hea_batch b, int32_t candidates, double tau, int32_t accepts) {
   if (!b) return fail(HEA_EINVAL, "null batch");
   int rc;
   for (auto q : b->probs)
-    if ((rc = check_guide(q, candidates, tau))) return rc;
+    if ((rc = check_guide(q, candidates, tau, accepts))) return rc;
   HEA_HIP(hipSetDevice(b->probs[0]->device));
   const int P = (int)b->probs.size();
   size_t n = 0;
@@ -1223,11 +1227,13 @@ int hea_batch_guide(hea_batch b, int32_t candidates, double tau) {
     a.hist = base + off[q];
     a.g_cand = candidates;
     a.g_tau = tau;
+    a.g_multi = accepts;
   });
   if (rc) return rc;
   b->guide.hist = b->hist.p;
   b->guide.cand = candidates;
   b->guide.tau = tau;
+  b->guide.multi = accepts;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);

@@ -56,6 +56,7 @@ struct HeaMcArgs {
   uint32_t* hist;               // [L][N][4] packed neighbour species counts per shell
   int32_t g_cand;               // candidate pairs per proposal
   double g_tau;                 // softmax temperature in units of the gain scale
+  int32_t g_multi;              // accepts per round at most
   // problem scalars, copied from HeaConst so they live in the kernarg segment
   int32_t N, K, S, stride;
   int32_t Z[HEA_MAX_S], off[HEA_MAX_S];

@@ -61,6 +61,11 @@ namespace {
 
 constexpr int kMaxWaves = 16;  // 1024 lanes
 constexpr double kU = 0x1p-53;  // unit roundoff of binary64
+constexpr int kMapWords = 1024;  // 32768-bit conflict map (guided mode)
+
+// Bit of site i in the conflict map (Fibonacci hash; collisions only make
+// the map more conservative).
+__device__ __forceinline__ uint32_t map_bit(int i) { return ((uint32_t)i * 0x9e3779b1u) >> 17; }
 
 struct McShared {
   int wcount[kMaxWaves];
@@ -71,6 +76,7 @@ struct McShared {
   unsigned long long res_key;           // exact stage: winner's fitness bits
   int res_j, res_a, res_b;
   uint32_t next_pos;
+  int ncl;        // entries of the candidate list (a superset, see combine step)
   unsigned gmax;  // guided mode: float bits of max_s Z_s |H_s| (gain scale)
 };
 
@@ -217,7 +223,7 @@ template <int KMAX>
 struct LdsLayout {
   static constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
   using PT = typename Pack<KMAX>::T;
-  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, gkey, WA, WB, WC, cl, sh, sp, total;
+  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, gkey, cmap, WA, WB, WC, cl, sh, sp, total;
   __host__ __device__ LdsLayout(int task, int n, bool sp_lds) {
     size_t o = 0;
     auto take = [&o](size_t bytes) {
@@ -230,13 +236,14 @@ struct LdsLayout {
     R = take(sizeof(double) * TAB);
     tgt = take(sizeof(double) * TAB);
     SC = take(sizeof(double2) * TAB);
-    H = take(sizeof(float) * TAB);
+    H = take(sizeof(float) * HEA_MAX_S * KMAX * 8);
     conc = take(sizeof(double) * KMAX);
     xs = take(sizeof(double) * HEA_MAX_S * (size_t)task);
     ex = take(sizeof(double) * 64);
     xw = take(sizeof(double) * 64 * (4 + KMAX));
     prop = take(sizeof(int2) * (size_t)task);
     gkey = take(sizeof(unsigned long long) * (size_t)task);
+    cmap = take(sizeof(uint32_t) * kMapWords);
     WA = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
     WB = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
     WC = take(sizeof(uint32_t) * HEA_MAX_S * (size_t)task);
@@ -256,7 +263,7 @@ struct Lds {
   double* R;     // [S][KMAX][KMAX] safe step or 0
   double* tgt;   // [S][KMAX][KMAX] target alpha
   double2* SC;   // [S][KMAX][KMAX] {d_ij + d_ji, -delta_s/dd_ij}, d = alpha - t
-  float* H;      // [S][KMAX][KMAX] guided mode: dF2/dn + its transpose
+  float* H;      // [S][KMAX][8] guided mode: dF2/dn + its transpose (rows padded to 8)
   double* conc;  // [KMAX] fl(count/N)
   double* xs;    // [S][task] screen values; row 0 then holds the combined X
   double* ex;    // [64] exact-stage shell sums (wave 0)
@@ -264,6 +271,7 @@ struct Lds {
                  // their alphas, then the terms of row x
   int2* prop;    // [task]
   unsigned long long* gkey;  // [task] guided mode: best (score, candidate) of each slot
+  uint32_t* cmap;  // [kMapWords] guided mode: hashed sites touched by this round's accepts
   PT* WA;        // [S][task] walk results: histogram of a1's neighbours
   PT* WB;        // [S][task] histogram of a2's neighbours
   uint32_t* WC;  // [S][task] c11 | c12 << 8 | c21 << 16 | c22 << 24
@@ -285,6 +293,7 @@ struct Lds {
     xw = reinterpret_cast<double*>(base + L.xw);
     prop = reinterpret_cast<int2*>(base + L.prop);
     gkey = reinterpret_cast<unsigned long long*>(base + L.gkey);
+    cmap = reinterpret_cast<uint32_t*>(base + L.cmap);
     WA = reinterpret_cast<PT*>(base + L.WA);
     WB = reinterpret_cast<PT*>(base + L.WB);
     WC = reinterpret_cast<uint32_t*>(base + L.WC);
@@ -293,7 +302,14 @@ struct Lds {
     sp = sp_lds ? base + L.sp : nullptr;
   }
   __device__ __forceinline__ int idx(int s, int i, int j) const { return (s * KMAX + i) * KMAX + j; }
+  __device__ __forceinline__ int hidx(int s, int i, int j) const { return (s * KMAX + i) * 8 + j; }
 };
+
+template <int KMAX>
+__device__ __forceinline__ bool touched(const Lds<KMAX>& T, int2 pr) {
+  const uint32_t b1 = map_bit(pr.x), b2 = map_bit(pr.y);
+  return ((T.cmap[b1 >> 5] >> (b1 & 31u)) | (T.cmap[b2 >> 5] >> (b2 & 31u))) & 1u;
+}
 
 // Orders LDS traffic between the lanes of one wave (the hardware keeps a
 // wave's LDS operations in order; this stops the compiler from moving them).
@@ -359,7 +375,7 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
     // (accelerate.cpp:556-569) with gamma = n delta: dF2/dn_ij =
     // -2 w (alpha_ij - t_ij) delta / dd_ij = 2 w d_ij c_ij, c_ij = -delta/dd_ij
     const float h = (float)(2.0 * A.weight[s] * (dij + dji) * T.SC[e].y);
-    T.H[e] = h;
+    T.H[T.hidx(s, i, j)] = h;
     atomicMax(&T.sh->gmax, __float_as_uint(fabsf(h) * (float)A.Z[s]));
   }
 }
@@ -518,8 +534,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     T.sh->ubmax[tid] = 0ull;
   }
   if (tid == 0) T.sh->gmax = 0u;
-  if constexpr (GUIDED)
+  if constexpr (GUIDED) {
     for (int j = tid; j < task; j += nthr) T.gkey[j] = 0ull;
+    // absent shells and species keep zero gains
+    for (int e = tid; e < HEA_MAX_S * KMAX * 8; e += nthr) T.H[e] = 0.0f;
+  }
   if (tid < kMaxWaves) {  // slots of absent waves stay neutral
     T.sh->wcount[tid] = 0;
     T.sh->wmin[tid] = __builtin_inf();
@@ -535,7 +554,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   const uint32_t iter = *A.iter_ptr;
   uint32_t pos = 0;
   int acc = 0;
-  long long rounds = 0, accepts = 0;
+  long long rounds = 0, accepts = 0, screens = 0;
   bool hit_thr = false;
 
   constexpr bool guided = GUIDED;
@@ -559,8 +578,6 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
                       hea_mix32((uint32_t)l + 0x3c6ef372u);
 
   while (acc < A.depth) {
-    // window of this round's tables; published by the barriers below
-    if (tid == 0) T.sh->beta = screen_beta<KMAX>(A, T);
     if constexpr (GUIDED) {
       // ---- guided: slot j takes the Gumbel-max of -gain/(tau*scale) over
       // g_cand hashed candidate pairs (same-species pairs skipped); all lanes
@@ -595,23 +612,20 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (!live[q]) continue;
           const int a = sa[q], b = sb[q];
           const uint32_t hh1[3] = {(uint32_t)hv[q][0], (uint32_t)(hv[q][0] >> 32), (uint32_t)hv[q][1]};
           const uint32_t hh2[3] = {(uint32_t)hv[q][2], (uint32_t)(hv[q][2] >> 32), (uint32_t)hv[q][3]};
-          // first-order change of F^2: sum_s sum_d (H_s[b][d] - H_s[a][d]) (cnt1 - cnt2)
+          // first-order change of F^2: sum_s sum_d (H_s[b][d] - H_s[a][d]) (cnt1 - cnt2);
+          // absent shells and species have zero rows and zero counts
           float d = 0.0f;
 #pragma unroll
           for (int sh = 0; sh < HEA_MAX_S; ++sh) {
-            if (sh < S) {
-              const float* Ha = T.H + T.idx(sh, a, 0);
-              const float* Hb = T.H + T.idx(sh, b, 0);
+            const float* Ha = T.H + T.hidx(sh, a, 0);
+            const float* Hb = T.H + T.hidx(sh, b, 0);
 #pragma unroll
-              for (int x = 0; x < KMAX; ++x)
-                if (EXACT || x < K)
-                  d += (Hb[x] - Ha[x]) * (float)((int)((hh1[sh] >> (5 * x)) & 31u) -
-                                                 (int)((hh2[sh] >> (5 * x)) & 31u));
-            }
+            for (int x = 0; x < KMAX; ++x)
+              d += (Hb[x] - Ha[x]) * (float)((int)((hh1[sh] >> (5 * x)) & 31u) -
+                                             (int)((hh2[sh] >> (5 * x)) & 31u));
           }
           const float U = ((float)(h3[q] >> 8) + 0.5f) * 0x1p-24f;
           const float sc = -d * inv - __logf(-__logf(U));
@@ -620,7 +634,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           const uint32_t ob = (fb & 0x80000000u) ? ~fb : (fb | 0x80000000u);
           const int w = w0 + q * nthr;
           const int j = w / g, c = w - j * g;
-          atomicMax(&T.gkey[j], ((unsigned long long)ob << 32) | (uint32_t)(0xffffffffu - (uint32_t)c));
+          if (live[q])
+            atomicMax(&T.gkey[j], ((unsigned long long)ob << 32) | (uint32_t)(0xffffffffu - (uint32_t)c));
+          __builtin_amdgcn_sched_barrier(0);  // one item's gain rows at a time
         }
       }
       __syncthreads();
@@ -666,13 +682,35 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       if (found >= task) pos = T.sh->next_pos;
     }
 
+    // Guided mode accepts up to g_multi swaps per round: after an accept,
+    // the proposals that touch none of this round's swapped sites or their
+    // neighbours keep valid walks and are screened again against the
+    // refreshed tables; the best goes through the exact stage.
+    if constexpr (GUIDED)
+      for (int e = tid; e < kMapWords; e += nthr) T.cmap[e] = 0u;
+    for (int sub = 0;; ++sub) {
+    ++screens;
+    // window of the current tables and an empty candidate list; the screen's
+    // barrier publishes both
+    if (tid == 0) {
+      T.sh->beta = screen_beta<KMAX>(A, T);
+      T.sh->ncl = 0;
+    }
     // ---- screen: walk both neighbour lists of every proposal, keep the walk
     for (int q = 0; q < nq; ++q) {
       const int j = jl + q * TP;
       if (j < task) {
         const int2 pr = T.prop[j];
         double x = 0.0;
-        if (pr.x >= 0) {
+        if (GUIDED && sub > 0) {
+          if (pr.x >= 0 && !touched(T, pr)) {  // walk stored by the first screen
+            const int w = s_me * task + j;
+            const uint32_t cp = T.WC[w];
+            x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, sp[pr.x], sp[pr.y], T.WA[w],
+                                          T.WB[w], (int)(cp & 255u), (int)((cp >> 8) & 255u),
+                                          (int)((cp >> 16) & 255u), (int)(cp >> 24));
+          }
+        } else if (pr.x >= 0) {
           const int a = sp[pr.x], b = sp[pr.y];
           const int32_t* r1 = A.nbr + (size_t)pr.x * A.stride + A.off[s_me];
           const int32_t* r2 = A.nbr + (size_t)pr.y * A.stride + A.off[s_me];
@@ -695,7 +733,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     // ---- X = sum_s w_s x_s and its minimum
     double lmin = __builtin_inf();
     for (int j = tid; j < task; j += nthr) {
-      if (T.prop[j].x >= 0) {
+      const int2 pr = T.prop[j];
+      if (pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr))) {
         double X = 0.0;
         for (int s = 0; s < S; ++s) X = X + A.weight[s] * T.xs[s * task + j];
         T.xs[j] = X;
@@ -705,6 +744,26 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) lmin = fmin(lmin, __shfl_xor(lmin, m));
     if (lane == 0) T.sh->wmin[wave] = lmin;
+    {
+      // this wave's proposals within 2 beta of the wave's minimum: a superset
+      // of the final candidates, since the workgroup's minimum is no larger
+      const double wthr = lmin + 2.0 * T.sh->beta;
+      for (int j0 = 0; j0 < task; j0 += nthr) {
+        const int j = j0 + tid;
+        bool f = false;
+        if (j < task) {
+          const int2 pr = T.prop[j];
+          f = pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr)) && !(T.xs[j] > wthr);
+        }
+        const unsigned long long bal = __ballot(f);
+        if (bal) {
+          int base = 0;
+          if (lane == 0) base = atomicAdd(&T.sh->ncl, __popcll(bal));
+          base = __builtin_amdgcn_readfirstlane(base);
+          if (f) T.cl[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+        }
+      }
+    }
     __syncthreads();
 
     // ---- exact stage (wave 0): candidates within the window, first strict
@@ -715,18 +774,16 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       for (int w = 1; w < kMaxWaves; ++w) xmin = fmin(xmin, T.sh->wmin[w]);
       double thr = xmin + 2.0 * T.sh->beta;
       if (!(thr < __builtin_inf())) thr = __builtin_inf();
-      const int nk = (task + 63) >> 6;  // <= 32
-      uint32_t fl = 0u;                  // bit k: slot k*64 + lane is a candidate
-#pragma unroll 8
-      for (int k = 0; k < nk; ++k) {
-        const int j = k * 64 + lane;
-        if (j < task && T.prop[j].x >= 0 && !(T.xs[j] > thr)) fl |= 1u << k;
-      }
+      // filter the waves' lists against the workgroup's window, compacting
+      // in place (every entry is read before any of its chunk is written)
+      const int nsup = T.sh->ncl;
       int nc = 0;
-      for (int k = 0; k < nk; ++k) {
-        const bool f = (fl >> k) & 1u;
+      for (int k0 = 0; k0 < nsup; k0 += 64) {
+        const int j = (k0 + lane < nsup) ? T.cl[k0 + lane] : -1;
+        const bool f = j >= 0 && !(T.xs[j] > thr);
         const unsigned long long bal = __ballot(f);
-        if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = k * 64 + lane;
+        wave_sync();
+        if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = j;
         nc += __popcll(bal);
       }
       wave_sync();
@@ -867,7 +924,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     }
     __syncthreads();
     const double best_fit = __longlong_as_double((long long)T.sh->res_key);
-    ++rounds;
+    if (sub == 0) ++rounds;
 
     // ---- accept iff strictly better (accelerate.cpp:678-689)
     if (best_fit < prev_fit) {
@@ -894,6 +951,12 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           const uint32_t dv = t < zs ? (1u << (5 * b)) - (1u << (5 * a)) : (1u << (5 * a)) - (1u << (5 * b));
           __hip_atomic_fetch_add(hist + (size_t)nb * 4 + s, dv, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint32_t mb = map_bit(nb);
+          atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
+        }
+        if (tid < 2) {
+          const uint32_t mb = map_bit(tid == 0 ? pr.x : pr.y);
+          atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
         }
       }
       for (int e = tid; e < S * K * K; e += nthr) {
@@ -905,8 +968,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         atomicMin(A.first_hit, (unsigned long long)wall_clock64());
         hit_thr = false;
       }
+      if (!GUIDED || sub + 1 >= A.g_multi) break;
     } else {
-      ++acc;
+      if (sub == 0) ++acc;
+      break;
+    }
     }
   }
 
@@ -918,7 +984,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     int64_t* st = A.stats + (size_t)l * 4;
     st[0] += rounds;
     st[1] += accepts;
-    st[2] += rounds * (long long)task;
+    st[2] += screens * (long long)task;
     st[3] += (long long)pos;
   }
 }

@@ -1,7 +1,9 @@
 """Guided proposals: the acceptance rule stays exact, so every stored
 fitness must agree with the oracle's recount of the returned lattice (to
 gamma-drift precision), composition is preserved, the population is sorted,
-and a run is reproducible from its seed."""
+and a run is reproducible from its seed.  With several accepts per round the
+extra swaps are checked the same way (they are accepted only on an exact
+fitness decrease, from proposals whose walks the round's swaps left valid)."""
 import numpy as np
 import pytest
 
@@ -17,10 +19,10 @@ def _spec(typ="FCC", cell=4, K=4, seed=11):
     return make_problem(typ, cell, fr, shells="true", target_seed=seed)
 
 
-def _run(prob, guided, L=8, I=3, T=64, D=3, seed=5):
+def _run(prob, guided, L=8, I=3, T=64, D=3, seed=5, accepts=1):
     s = prob.session(L, T, D, seed=seed)
     if guided:
-        s.guide(candidates=8, tau=0.1)
+        s.guide(candidates=8, tau=0.1, accepts=accepts)
     s.init()
     s.step(I)
     out = s.read()
@@ -28,17 +30,20 @@ def _run(prob, guided, L=8, I=3, T=64, D=3, seed=5):
     return out
 
 
+@pytest.mark.parametrize("accepts", [1, 4])
 @pytest.mark.parametrize("typ,cell,K", [("FCC", 4, 4), ("BCC", 5, 5), ("FCC", 3, 6)])
-def test_guided_exact_acceptance(gpu, typ, cell, K):
+def test_guided_exact_acceptance(gpu, typ, cell, K, accepts):
     spec = _spec(typ, cell, K)
     prob, o = Problem.from_spec(spec), Oracle.from_spec(spec)
-    lat, fit, st = _run(prob, True)
+    lat, fit, st = _run(prob, True, accepts=accepts)
     assert np.all(np.diff(fit) >= 0)
     assert np.allclose(o.fitness(lat), fit, rtol=1e-11, atol=0)
     for row in lat:
         assert np.array_equal(np.bincount(row, minlength=K), spec.counts)
-    assert st["evals"] == 64 * st["rounds"] and st["accepts"] > 0
-    lat2, fit2, _ = _run(prob, True)
+    assert st["evals"] % 64 == 0 and st["evals"] >= 64 * st["rounds"] and st["accepts"] > 0
+    if accepts == 1:
+        assert st["evals"] == 64 * st["rounds"]
+    lat2, fit2, _ = _run(prob, True, accepts=accepts)
     assert np.array_equal(lat, lat2) and np.array_equal(fit, fit2)
 
 
@@ -46,12 +51,12 @@ def test_guided_batch_matches_guided_sessions(gpu):
     specs = [_spec("FCC", 3, 4, 21), _spec("BCC", 4, 5, 22)]
     probs = [Problem.from_spec(s) for s in specs]
     b = Batch(probs, 4, 32, 2, seed=70)
-    b.guide(candidates=8, tau=0.1)
+    b.guide(candidates=8, tau=0.1, accepts=3)
     b.init()
     b.step(2)
     for q, p in enumerate(probs):
         s = p.session(4, 32, 2, seed=70 + q)
-        s.guide(candidates=8, tau=0.1)
+        s.guide(candidates=8, tau=0.1, accepts=3)
         s.init()
         s.step(2)
         ls, fs, _ = s.read()
