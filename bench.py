@@ -38,6 +38,21 @@ def bytes_per_eval(sum_z: int) -> int:
     return 2 * sum_z * (4 + 1) + 2
 
 
+def profiled_traffic(workload: str):
+    """HBM bytes per local-search launch from the committed rocprofv3 byte
+    counters (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known
+    size), or None when no profile covers the workload."""
+    path = os.path.join(ROOT, "profiles", f"k_local_mc_{workload.replace('-', '_')}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,7 +82,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int):
+def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0=None, fit0=None):
     """Reference engine (or the oracle when the reference binary is absent)
     running local search I on lattices of the same workload, one process per
     core.  Eval counts come from the GPU run of the identical work (the two
@@ -79,13 +94,17 @@ def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int):
 
     o = Oracle.from_spec(spec)
     kind = "reference" if RefHarness.available() else "port"
-    # per-lattice cost of local search I measured on the reference engine
-    # in this repository's container: ~0.6 s for BCC 16^3, T = 256
-    per_lat = 0.6
-    n_lat = max(procs, int(round(target_secs / per_lat)))
-    n_lat = (n_lat + procs - 1) // procs * procs
-    lat0 = np.stack([o.permute(run["seed"], 0, 1, l) for l in range(n_lat)])
-    fit0 = o.fitness(lat0)
+    start = "random"
+    if lat0 is None:
+        # per-lattice cost of local search I measured on the reference engine
+        # in this repository's container: ~0.6 s for BCC 16^3, T = 256
+        per_lat = 0.6
+        n_lat = max(procs, int(round(target_secs / per_lat)))
+        n_lat = (n_lat + procs - 1) // procs * procs
+        lat0 = np.stack([o.permute(run["seed"], 0, 1, l) for l in range(n_lat)])
+        fit0 = o.fitness(lat0)
+    else:
+        n_lat, start = len(lat0), "converged"
     T, D = run["parallel_task"], run["converge_depth"]
     chunks = np.array_split(np.arange(n_lat), procs)
     with tempfile.TemporaryDirectory() as tmp:
@@ -108,7 +127,7 @@ def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int):
         "unit": "swap evals/s",
         "cores": procs,
         "kind": kind,
-        "sample": f"local search I (T={T}, depth={D}) on {n_lat} random "
+        "sample": f"local search I (T={T}, depth={D}) on {n_lat} {start} "
                   f"{spec.latt_typ} {spec.cell_dim}^3 lattices, "
                   f"{procs} processes, {st['evals']} evals in {wall:.1f} s",
     }
@@ -155,7 +174,9 @@ def time_to_fitness(a):
             trace.append((round(time.perf_counter() - t0, 3), best))
             if best <= a.thr or time.perf_counter() - t0 > a.budget_s:
                 break
-        _, fit, st = s.read()
+        lat_f, fit, st = s.read()
+        if mode == "plain":
+            plain_final = (lat_f, fit)
         modes[mode] = {"best": float(fit[0]), "iterations": len(trace), "wall_s": trace[-1][0],
                        "evals": st["evals"], "evals_per_s": st["evals"] / trace[-1][0],
                        "time_to_thr_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,
@@ -168,7 +189,75 @@ def time_to_fitness(a):
     tp, tg = modes["plain"]["time_to_plain_best_s"], modes["guided"]["time_to_plain_best_s"]
     out = {"metric": "time to fitness (guided vs plain)", "workload": a.workload,
            "n_sites": spec.n_sites, "solutions": L, "parallel_task": T, "thr": a.thr,
+           "guide": {"candidates": a.guide_candidates, "tau": 0.1, "accepts": a.guide_accepts},
            "speedup_to_plain_best": (tp / tg) if (tp and tg) else None, "modes": modes}
+    hp, hg = modes["plain"]["time_to_thr_s"], modes["guided"]["time_to_thr_s"]
+    if hp and hg:
+        out["speedup_to_thr"] = hp / hg
+    if a.cpu_seconds > 0 and hp and hg:
+        # the reference engine makes plain mode's decisions, so it needs the
+        # evals plain mode needed; its rate is timed on lattices of the plain
+        # run's final population (bounded: they are converged)
+        procs = a.cpu_procs
+        cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, procs,
+                           lat0=plain_final[0][:4 * procs], fit0=plain_final[1][:4 * procs])
+        pm = modes["plain"]
+        evals_to_thr = pm["evals"] * hp / pm["wall_s"]  # plain's rate is steady
+        cpu["time_to_thr_projected_s"] = evals_to_thr / cpu["value"]
+        out["cpu_baseline"] = cpu
+        out["speedup_guided_vs_cpu"] = cpu["time_to_thr_projected_s"] / hg
+    print(json.dumps(out))
+
+
+def batched(a):
+    """Batched workload: 64 problems of uneven size, species count and lattice
+    type (cell edges 6-20, FCC/BCC, K 3-6) searched together, 128 solutions
+    each, parallel_task 256, converge_depth 10; one persistent local-search
+    launch per phase, the iteration replayed as a HIP graph.  Launch shape
+    from the autotuner.  Algorithmic bytes per eval follow each problem's
+    shell sizes; the rate is over whole iterations (every kernel of the graph
+    counts), so it bounds the local-search kernel's rate from below."""
+    from pyhea_amd.hcs import Batch, Problem
+    from pyhea_amd.problem import batched_workload
+
+    specs = batched_workload()
+    probs = [Problem.from_spec(sp) for sp in specs]
+    L, T, D, seed = 128, 256, 10, 2002
+    shape = {"grid": None, "tile": None}
+    if not a.no_tune:
+        from pyhea_amd.autotune import tune_batch
+
+        shape = tune_batch(probs, L, T, D, seed=seed, cache_path=None)
+    b = Batch(probs, L, T, D, seed=seed)
+    if shape["tile"]:
+        b.tile(shape["tile"])
+    if shape["grid"]:
+        b.set_grid(shape["grid"])
+    b.init()
+    b.step(a.warmup)
+    b.sync()
+    ev0 = [b.read(q)[2]["evals"] for q in range(len(probs))]
+    t0 = time.perf_counter()
+    b.step(a.steps)
+    b.sync()
+    dt = time.perf_counter() - t0
+    ev1 = [b.read(q)[2]["evals"] for q in range(len(probs))]
+    evals = [e1 - e0 for e0, e1 in zip(ev0, ev1)]
+    total = sum(evals)
+    byts = sum(e * bytes_per_eval(sum(sp.shell_sizes[:sp.S])) for e, sp in zip(evals, specs))
+    b.close()
+    out = {
+        "metric": "swap evals/s", "value": total / dt, "unit": "swap evals/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched-64", "problems": len(specs),
+                   "sites": [sp.n_sites for sp in specs], "solutions": L, "parallel_task": T,
+                   "converge_depth": D, "seed": seed, "grid": shape["grid"], "tile": shape["tile"]},
+        "roofline": {"bound": "hbm", "achieved": round(byts / dt / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(byts / dt / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_local_mc_batch (whole iterations)"},
+    }
     print(json.dumps(out))
 
 
@@ -177,6 +266,10 @@ def main():
         cpu_worker(sys.argv[2], sys.argv[3], *map(int, sys.argv[4:8]))
         return
     a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the population would shard one sub-population per GPU)"}))
+        sys.exit(2)
     if a.dump_problem:
         from pyhea_amd.problem import workload
 
@@ -193,10 +286,9 @@ def main():
     if a.workload.startswith("ttf-"):
         time_to_fitness(a)
         return
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the population would shard one sub-population per GPU)"}))
-        sys.exit(2)
+    if a.workload == "batched-64":
+        batched(a)
+        return
     from pyhea_amd import _native
     from pyhea_amd.hcs import Problem
     from pyhea_amd.problem import workload
@@ -241,7 +333,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": profiled_traffic(a.workload),
         "kernel": "k_local_mc",
         "bytes_per_eval": bpe,
         "avg_launch_ms": round(float(ms.mean()), 4),

@@ -2,13 +2,16 @@
 // C ABI with no Python in the process, so hardware-counter profilers can
 // attach to a plain executable.
 //
-//   hea_profile <problem.bin> <solutions> <task> <depth> <seed> <warmup> <steps> [copy_mib]
+//   hea_profile <problem.bin> <solutions> <task> <depth> <seed> <warmup> <steps>
+//               [copy_mib [tile [candidates accepts]]]
 //
 // problem.bin (written by bench.py --dump-problem): int32 N, K, S, Z[3];
 // int32 nbr[N][sum Z]; int32 counts[K]; float64 weight[S];
 // float64 target[S][K*K].  After the timed steps an optional device-to-device
 // copy of copy_mib MiB gives a kernel of known size for calibrating byte
-// counters.  Prints one line: evals, seconds, evals/s.
+// counters (0: none).  tile > 0 sets the proposals per pass; candidates > 0
+// switches to guided proposals (tau 0.1).  Prints one line: evals, seconds,
+// evals/s.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -35,7 +38,8 @@ static void rd(FILE* f, T* p, size_t n) {
 
 int main(int argc, char** argv) {
   if (argc < 8) {
-    fprintf(stderr, "usage: hea_profile problem.bin L T D seed warmup steps [copy_mib]\n");
+    fprintf(stderr,
+            "usage: hea_profile problem.bin L T D seed warmup steps [copy_mib [tile [cand acc]]]\n");
     return 2;
   }
   FILE* f = fopen(argv[1], "rb");
@@ -68,6 +72,9 @@ int main(int argc, char** argv) {
   need(hea_problem_create(&d, 0, &p), "problem");
   hea_session s;
   need(hea_session_create(p, L, T, D, 1e-3, seed, &s), "session");
+  if (argc > 9 && atoi(argv[9]) > 0) need(hea_session_tile(s, atoi(argv[9])), "tile");
+  if (argc > 11 && atoi(argv[10]) > 0)
+    need(hea_session_guide(s, atoi(argv[10]), 0.1, atoi(argv[11])), "guide");
   need(hea_session_init(s), "init");
   need(hea_session_step(s, warm), "warmup");
   need(hea_session_sync(s), "sync");
@@ -80,7 +87,7 @@ int main(int argc, char** argv) {
   need(hea_session_read(s, nullptr, nullptr, &st1), "read");
   const long long ev = st1.evals - st0.evals;
   printf("evals %lld seconds %.6f evals_per_s %.6e\n", ev, secs, ev / secs);
-  if (argc > 8) {
+  if (argc > 8 && atoi(argv[8]) > 0) {
     const size_t bytes = (size_t)atoi(argv[8]) << 20;
     void *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&b, bytes) == hipSuccess) {
