@@ -59,10 +59,50 @@ __global__ void k_zero_i32(int32_t* __restrict__ p, size_t n) {
 }
 
 constexpr int kRecountThreads = 256;
-constexpr int kRecountSites = 2048;  // sites per workgroup
+constexpr int kRecountSites = 2048;  // sites per workgroup (lattices beyond LDS)
+constexpr int kRecountLdsThreads = 1024;
 
-// Integer pair counts n[l][s][a][b]: per site, a packed histogram of the
-// neighbour species per shell, added into an LDS table, then into HBM.
+// Species histogram (8-bit fields, Z <= 31) of one shell's neighbour list of
+// NC = ceil(z/4) 16-byte chunks, all loaded before any is used.  Lists are
+// padded to a multiple of 4 entries.
+template <int NC>
+__device__ __forceinline__ uint64_t shell_hist(const int32_t* __restrict__ row, int z,
+                                               const uint8_t* __restrict__ sp) {
+  int4 u[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) u[c] = reinterpret_cast<const int4*>(row)[c];
+  uint64_t m = 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int nb[4] = {u[c].x, u[c].y, u[c].z, u[c].w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool ok = (c < NC - 1) || (4 * c + t < z);  // uniform
+      const uint64_t f = 1ull << (8 * sp[ok ? nb[t] : 0]);
+      m += ok ? f : 0ull;
+    }
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint64_t shell_hist(const int32_t* __restrict__ row, int z,
+                                               const uint8_t* __restrict__ sp) {
+  switch ((z + 3) >> 2) {  // uniform
+    case 1: return shell_hist<1>(row, z, sp);
+    case 2: return shell_hist<2>(row, z, sp);
+    case 3: return shell_hist<3>(row, z, sp);
+    case 4: return shell_hist<4>(row, z, sp);
+    case 5: return shell_hist<5>(row, z, sp);
+    case 6: return shell_hist<6>(row, z, sp);
+    case 7: return shell_hist<7>(row, z, sp);
+    default: return shell_hist<8>(row, z, sp);
+  }
+}
+
+// Integer pair counts n[s][a][b] of every lattice, added to `counts` (which
+// the caller zeroes): the count form of calculate_sro_coefficient
+// (accelerate.cpp:370-399).  Several workgroups per lattice; species read
+// from HBM.  Used for lattices that do not fit in LDS.
 __global__ __launch_bounds__(kRecountThreads) void k_recount(
     const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
     const uint8_t* __restrict__ lattices, int32_t* __restrict__ counts) {
@@ -79,8 +119,7 @@ __global__ __launch_bounds__(kRecountThreads) void k_recount(
     const int a = lat[i];
     const int32_t* row = nbr + (size_t)i * P.stride;
     for (int s = 0; s < S; ++s) {
-      uint64_t m = 0;  // 8-bit fields, Z <= 31
-      for (int k = 0; k < P.Z[s]; ++k) m += 1ull << (8 * lat[row[P.off[s] + k]]);
+      const uint64_t m = shell_hist(row + P.off[s], P.Z[s], lat);
       for (int x = 0; x < K; ++x) {
         const int f = (int)((m >> (8 * x)) & 255u);
         if (f) atomicAdd(&hist[(s * K + a) * K + x], f);
@@ -91,6 +130,49 @@ __global__ __launch_bounds__(kRecountThreads) void k_recount(
   int32_t* dst = counts + (size_t)l * S * K * K;
   for (int e = threadIdx.x; e < S * K * K; e += blockDim.x)
     if (hist[e]) atomicAdd(&dst[e], hist[e]);
+}
+
+// The same with one workgroup per lattice and the lattice's species staged
+// in LDS; per-wave histogram tables, then one store per entry (no zeroing of
+// `counts` needed).
+__global__ __launch_bounds__(kRecountLdsThreads) void k_recount_lds(
+    const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
+    const uint8_t* __restrict__ lattices, int32_t* __restrict__ counts) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int TAB = HEA_MAX_S * HEA_MAX_K * HEA_MAX_K;
+  const HeaConst& P = *pc;
+  const int l = blockIdx.x;
+  const int K = P.K, S = P.S, N = P.N, SKK = S * K * K;
+  const int tid = threadIdx.x, wave = tid >> 6, nw = blockDim.x >> 6;
+  int32_t* wh = reinterpret_cast<int32_t*>(smem);  // [nw][TAB]
+  uint8_t* sp = smem + sizeof(int32_t) * TAB * (kRecountLdsThreads / 64);
+  const uint8_t* lat = lattices + (size_t)l * N;
+  for (int i = tid; i < N; i += blockDim.x) sp[i] = lat[i];
+  for (int e = tid; e < nw * TAB; e += blockDim.x) wh[e] = 0;
+  __syncthreads();
+  int32_t* my = wh + wave * TAB;
+  for (int i = tid; i < N; i += blockDim.x) {
+    const int a = sp[i];
+    const int32_t* row = nbr + (size_t)i * P.stride;
+    for (int s = 0; s < S; ++s) {
+      const uint64_t m = shell_hist(row + P.off[s], P.Z[s], sp);
+      for (int x = 0; x < K; ++x) {
+        const int f = (int)((m >> (8 * x)) & 255u);
+        if (f) atomicAdd(&my[(s * K + a) * K + x], f);
+      }
+    }
+  }
+  __syncthreads();
+  int32_t* dst = counts + (size_t)l * SKK;
+  for (int e = tid; e < SKK; e += blockDim.x) {
+    int sum = 0;
+    for (int w = 0; w < nw; ++w) sum += wh[w * TAB + e];
+    dst[e] = sum;
+  }
+}
+
+size_t recount_lds_bytes(int n) {
+  return sizeof(int32_t) * HEA_MAX_S * HEA_MAX_K * HEA_MAX_K * (kRecountLdsThreads / 64) + n;
 }
 
 // Fitness of freshly recounted lattices: gamma = R_s(n), then the
@@ -304,6 +386,13 @@ int ensure_scratch(hea_problem p, int L) {
 
 int launch_recount(hea_problem p, const uint8_t* lat, int32_t* counts, int L, hipStream_t st) {
   const HeaConst& c = p->hc;
+  const size_t lds = recount_lds_bytes(c.N);
+  if ((int64_t)lds <= p->max_lds) {  // one workgroup per lattice, species in LDS
+    hipLaunchKernelGGL(k_recount_lds, dim3(L), dim3(kRecountLdsThreads), lds, st, p->d_const.p,
+                       p->d_nbr.p, lat, counts);
+    HEA_HIP(hipGetLastError());
+    return HEA_OK;
+  }
   const size_t cn = (size_t)L * c.S * c.K * c.K;
   hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)std::min<size_t>((cn + 255) / 256, 4096)),
                      dim3(256), 0, st, counts, cn);
@@ -534,6 +623,9 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
     e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
     p->max_lds = lds_max;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_recount_lds),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
     if (p->d_const.alloc(1) != hipSuccess || p->d_nbr.alloc(padded.size()) != hipSuccess) {
       rc = fail(HEA_ENOMEM, "device allocation failed");
       break;

@@ -359,8 +359,11 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
   // cover them)
   const double z = (double)A.Z[s], delta = A.delta[s];
   const double D1 = fabs(dij) + fabs(dji);
-  const double Ae = (6.0 * z * delta + (6.0 * z + 4.0) * 0x1p-52) / dd * 1.0001;
-  const double e2 = (12.0 * z * kU * delta + (6.0 * z + 4.0) * 0x1p-52) / dd * 1.0001;
+  // 1/dd from the screen coefficient |c| = fl(delta/dd) and 1/delta = Z N
+  // (each within a few u; the 1.0001 factors cover them)
+  const double idd = fabs(T.SC[e].y) * (z * (double)A.N);
+  const double Ae = (6.0 * z * delta + (6.0 * z + 4.0) * 0x1p-52) * idd * 1.0001;
+  const double e2 = (12.0 * z * kU * delta + (6.0 * z + 4.0) * 0x1p-52) * idd * 1.0001;
   const double m = 2.0 * fabs(q) + 2.0 + D1 + 3.0 * Ae;
   const double e1 = 2.0 * kU * m * 1.0001;
   const double t1 = D1 + Ae + e1;

@@ -81,3 +81,30 @@ def test_run_hcs_bit_exact(gpu, case):
     assert np.array_equal(_bits(fg), _bits(fo))
     assert np.array_equal(lg, lo)
     assert sg["evals"] == so["evals"] and sg["accepts"] == so["accepts"]
+
+
+# Equimolar two- and three-species lattices: many proposals tie exactly and,
+# near convergence, the best ones differ in the last bits, which is where the
+# screen's window must still hold the reference's choice.
+TIE_CASES = [
+    ("BCC", 6, [0.5, 0.5], "true", 3),
+    ("FCC", 4, [0.5, 0.5], "pyhea", 3),
+    ("FCC", 4, [1 / 3] * 3, "true", 3),
+    ("BCC", 5, [0.5, 0.5], "pyhea", 1),
+    ("FCC", 6, [0.25] * 4, "true", 3),
+]
+
+
+@pytest.mark.parametrize("case", TIE_CASES, ids=lambda c: f"{c[0]}{c[1]}K{len(c[2])}S{c[4]}")
+def test_local_mc_to_convergence_bit_exact(gpu, case):
+    spec = _spec(case, seed=13)
+    o, p = Oracle.from_spec(spec), Problem.from_spec(spec)
+    lat0 = np.stack([o.permute(31, 0, 0, l) for l in range(4)])
+    fit0 = o.fitness(lat0)
+    lo, fo, so = o.local_mc(lat0, fit0, 256, 8, seed=31, iter=0, phase=2)
+    lg, fg, sg = p.local_mc(lat0, fit0, 256, 8, seed=31, iter=0, phase=2)
+    assert np.array_equal(_bits(fg), _bits(fo))
+    assert np.array_equal(lg, lo)
+    for k in ("evals", "rounds", "accepts", "draws"):
+        assert sg[k] == so[k], k
+    assert so["rounds"] > 8 * 4  # every lattice ran past its first rejections
