@@ -75,6 +75,12 @@ struct McShared {
   double beta;                          // candidate window half-width of this round
   unsigned long long res_key;           // exact stage: winner's fitness bits
   int res_j, res_a, res_b;
+  int res_kind;                         // 0 reject, 1 exact accept, 2 accept decided by the window
+  // prev_fit and its state (wave 0 decides; see the exact stage):
+  // pfm 0: the stored fitness, not necessarily F of the recounted gamma;
+  // 1: F(gamma); 2: F(gamma), not yet computed.  fest ~ err of gamma.
+  double pf, fest;
+  int pfm;
   uint32_t next_pos;
   int ncl;        // entries of the candidate list (a superset, see combine step)
   unsigned gmax;  // guided mode: float bits of max_s Z_s |H_s| (gain scale)
@@ -492,6 +498,30 @@ __device__ __forceinline__ void guided_pair(uint32_t gk, uint32_t ctr, uint32_t 
   *h3 = h2;
 }
 
+// Fitness of the current gamma as the reference computes it
+// (accelerate.cpp:557-569): E0 holds the current terms; lane s < S adds its
+// shell's K*K terms in row-major order.  Called by all lanes of one wave.
+template <int KMAX, bool EXACT>
+__device__ __forceinline__ double current_fitness(const HeaMcArgs& A, const Lds<KMAX>& T,
+                                                  int lane) {
+  const int K = A.K, S = A.S;
+  if (lane < S) {
+    const double* e0 = T.E0 + T.idx(lane, 0, 0);
+    double es = 0.0;
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i)
+#pragma unroll
+      for (int y = 0; y < KMAX; ++y)
+        if (EXACT || (i < K && y < K)) es = es + e0[i * KMAX + y];
+    T.ex[lane] = es;
+  }
+  wave_sync();
+  double err = 0.0;
+  for (int t = 0; t < S; ++t) err = err + A.weight[t] * T.ex[t];
+  wave_sync();  // ex is reused
+  return sqrt(err);
+}
+
 __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned long long f2,
                                          int j2) {
   return (f1 < f2) || (f1 == f2 && j1 < j2);
@@ -553,7 +583,12 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   }
   __syncthreads();
 
-  double prev_fit = A.fitness[l];
+  if (tid == 0) {
+    const double f0 = A.fitness[l];
+    T.sh->pf = f0;
+    T.sh->pfm = 0;
+    T.sh->fest = f0 * f0;
+  }
   const uint32_t iter = *A.iter_ptr;
   uint32_t pos = 0;
   int acc = 0;
@@ -800,6 +835,28 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       const double* grp = T.xw + g * K * XW;  // lane (c, s, y) at grp + y * XW
       unsigned long long bk = ~0ull;
       int bj = 0x7fffffff, bc = 0, wc0 = -1;
+      // When prev_fit = F(gamma), the window decides most rounds: every
+      // proposal's err lies within beta of E + X (E common, X = 0 for the
+      // current lattice), and 2 beta keeps sqrt's rounding out of the way.
+      // One candidate with X < -2 beta is certainly accepted; X > 2 beta for
+      // the smallest X is certainly a reject.  Near the threshold, or
+      // otherwise, the exact path decides.
+      double prev_fit = T.sh->pf, fest = T.sh->fest;
+      int pfm = T.sh->pfm;
+      const double mg = 2.0 * T.sh->beta;
+      const bool near = A.thr > 0.0 && !(fest + xmin > A.thr * A.thr * 1.01);
+      int kind = -1;
+      if (pfm != 0 && !near) {
+        if (nc == 1 && xmin + mg < 0.0) kind = 2;
+        else if (xmin - mg > 0.0) kind = 0;
+      }
+      if (kind == -1 && pfm == 2) {  // the exact path compares with prev_fit
+        prev_fit = current_fitness<KMAX, EXACT>(A, T, lane);
+        pfm = 1;
+      }
+      if (kind == 0) {
+        if (lane == 0) T.sh->res_kind = 0;
+      } else {
       for (int c0 = 0; c0 < nc; c0 += cpc) {
         const int j = (lane < cpc * GS && c0 + c < nc) ? T.cl[c0 + c] : -1;
         int a = 0, b = 0;
@@ -819,6 +876,16 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           slot[3] = 1.0 - nB / (T.conc[b] * T.conc[x]);
         }
         wave_sync();
+        if (kind == 2) {  // the one candidate is accepted: its new rows only
+          if (j >= 0) {
+            T.G[T.idx(s, a, x)] = slot[0];
+            T.G[T.idx(s, x, a)] = slot[0];
+            T.G[T.idx(s, b, x)] = slot[1];
+            T.G[T.idx(s, x, b)] = slot[1];
+          }
+          bj = T.cl[0];
+          break;
+        }
         if (j >= 0) {  // terms of row x (accelerate.cpp:560-566)
           const double* tg = T.tgt + T.idx(s, x, 0);
           const double* e0 = T.E0 + T.idx(s, x, 0);
@@ -884,6 +951,22 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         }
         if (c0 + cpc < nc) wave_sync();  // the next chunk reuses xw and ex
       }
+      if (kind == 2) {
+        if (lane < HEA_MAX_S) {  // the refresh below takes new maxima
+          T.sh->bmax[lane] = 0ull;
+          T.sh->ubmax[lane] = 0ull;
+        }
+        if (lane == 0) {
+          const int2 pr = T.prop[bj];
+          T.sh->gmax = 0u;
+          T.sh->res_kind = 2;
+          T.sh->res_j = bj;
+          T.sh->res_a = sp[pr.x];
+          T.sh->res_b = sp[pr.y];
+          T.sh->pfm = 2;
+          T.sh->fest = fest + xmin;
+        }
+      } else {
       const double best_fit = __longlong_as_double((long long)bk);
       int a = 0, b = 0;
       if (best_fit < prev_fit) {  // accept: write the winner's rows
@@ -919,24 +1002,30 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         if (lane == 0) T.sh->gmax = 0u;
       }
       if (lane == 0) {
+        const bool acc1 = best_fit < prev_fit;
+        T.sh->res_kind = acc1 ? 1 : 0;
         T.sh->res_key = bk;
         T.sh->res_j = bj;
         T.sh->res_a = a;
         T.sh->res_b = b;
+        T.sh->pf = acc1 ? best_fit : prev_fit;
+        T.sh->pfm = acc1 ? 1 : pfm;
+        if (acc1) T.sh->fest = best_fit * best_fit;
+      }
+      }
       }
     }
     __syncthreads();
-    const double best_fit = __longlong_as_double((long long)T.sh->res_key);
+    const int kind = T.sh->res_kind;
     if (sub == 0) ++rounds;
 
     // ---- accept iff strictly better (accelerate.cpp:678-689)
-    if (best_fit < prev_fit) {
+    if (kind != 0) {
       const int2 pr = T.prop[T.sh->res_j];
       const int a = T.sh->res_a, b = T.sh->res_b;
-      prev_fit = best_fit;
+      if (kind == 1 && __longlong_as_double((long long)T.sh->res_key) <= A.thr) hit_thr = true;
       acc = 0;
       ++accepts;
-      if (best_fit <= A.thr) hit_thr = true;
       if (tid == 0) {
         sp[pr.x] = (uint8_t)b;
         sp[pr.y] = (uint8_t)a;
@@ -979,6 +1068,16 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     }
   }
 
+  double prev_fit = T.sh->pf;
+  if (T.sh->pfm == 2 && tid == 0) {  // F of the final gamma (accelerate.cpp:557-569)
+    double err = 0.0;
+    for (int t = 0; t < S; ++t) {
+      double es = 0.0;
+      for (int i = 0; i < K * K; ++i) es = es + T.E0[T.idx(t, i / K, i % K)];
+      err = err + A.weight[t] * es;
+    }
+    prev_fit = sqrt(err);
+  }
   if constexpr (SP_LDS) {
     for (int i = tid; i < N; i += nthr) lat_g[i] = sp[i];
   }
