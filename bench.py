@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--cpu-procs", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (default tile)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="plain workloads: proposals per pass (64..256) instead of the autotuner's")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a copy "
                          "kernel of known size for calibrating HBM byte counters)")
@@ -152,8 +154,11 @@ def cpu_worker(path: str, kind: str, T: int, D: int, seed: int, wid: int) -> Non
 def time_to_fitness(a):
     """Plain vs guided search on one problem: global iterations until the
     population best reaches --thr or --budget-s runs out; the device clock
-    stamps the first local-search accept at or below the threshold."""
-    from pyhea_amd.hcs import Problem
+    stamps the first local-search accept at or below the threshold.  Guided
+    search runs twice: through a session (one local-search launch per phase)
+    and through the batched persistent launch replayed as a HIP graph
+    ("guided-batched"); both make the same decisions."""
+    from pyhea_amd.hcs import Batch, Problem
     from pyhea_amd.problem import workload
 
     wid = {"ttf-fcc30": 3, "ttf-fcc50": 4}[a.workload]
@@ -161,9 +166,12 @@ def time_to_fitness(a):
     L, T, D, seed = run["solutions"], run["parallel_task"], run["converge_depth"], run["seed"]
     prob = Problem.from_spec(spec)
     modes = {}
-    for mode in ("plain", "guided"):
-        s = prob.session(L, T, D, thr=a.thr, seed=seed)
-        if mode == "guided":
+    for mode in ("plain", "guided", "guided-batched"):
+        if mode == "guided-batched":
+            s = Batch([prob], L, T, D, thr=a.thr, seed=seed)
+        else:
+            s = prob.session(L, T, D, thr=a.thr, seed=seed)
+        if mode != "plain":
             s.guide(candidates=a.guide_candidates, tau=0.1, accepts=a.guide_accepts)
         t0 = time.perf_counter()
         s.init()
@@ -174,10 +182,10 @@ def time_to_fitness(a):
             trace.append((round(time.perf_counter() - t0, 3), best))
             if best <= a.thr or time.perf_counter() - t0 > a.budget_s:
                 break
-        lat_f, fit, st = s.read()
+        lat_f, fit, st = s.read() if mode != "guided-batched" else s.read(0)
         if mode == "plain":
             plain_final = (lat_f, fit)
-        modes[mode] = {"best": float(fit[0]), "iterations": len(trace), "wall_s": trace[-1][0],
+        modes[mode] = {"best": float(fit.min()), "iterations": len(trace), "wall_s": trace[-1][0],
                        "evals": st["evals"], "evals_per_s": st["evals"] / trace[-1][0],
                        "time_to_thr_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,
                        "trace": trace}
@@ -186,14 +194,17 @@ def time_to_fitness(a):
     target = modes["plain"]["best"]
     for m in modes.values():
         m["time_to_plain_best_s"] = next((t for t, f in m["trace"] if f <= target), None)
-    tp, tg = modes["plain"]["time_to_plain_best_s"], modes["guided"]["time_to_plain_best_s"]
+    tp = modes["plain"]["time_to_plain_best_s"]
     out = {"metric": "time to fitness (guided vs plain)", "workload": a.workload,
            "n_sites": spec.n_sites, "solutions": L, "parallel_task": T, "thr": a.thr,
            "guide": {"candidates": a.guide_candidates, "tau": 0.1, "accepts": a.guide_accepts},
-           "speedup_to_plain_best": (tp / tg) if (tp and tg) else None, "modes": modes}
-    hp, hg = modes["plain"]["time_to_thr_s"], modes["guided"]["time_to_thr_s"]
-    if hp and hg:
-        out["speedup_to_thr"] = hp / hg
+           "modes": modes}
+    hp = modes["plain"]["time_to_thr_s"]
+    for m in ("guided", "guided-batched"):
+        tg, hg = modes[m]["time_to_plain_best_s"], modes[m]["time_to_thr_s"]
+        out[f"speedup_to_plain_best_{m}"] = (tp / tg) if (tp and tg) else None
+        out[f"speedup_to_thr_{m}"] = (hp / hg) if (hp and hg) else None
+    hg = modes["guided-batched"]["time_to_thr_s"]
     if a.cpu_seconds > 0 and hp and hg:
         # the reference engine makes plain mode's decisions, so it needs the
         # evals plain mode needed; its rate is timed on lattices of the plain
@@ -205,7 +216,7 @@ def time_to_fitness(a):
         evals_to_thr = pm["evals"] * hp / pm["wall_s"]  # plain's rate is steady
         cpu["time_to_thr_projected_s"] = evals_to_thr / cpu["value"]
         out["cpu_baseline"] = cpu
-        out["speedup_guided_vs_cpu"] = cpu["time_to_thr_projected_s"] / hg
+        out["speedup_guided_batched_vs_cpu"] = cpu["time_to_thr_projected_s"] / hg
     print(json.dumps(out))
 
 
@@ -300,8 +311,8 @@ def main():
     spec, run = workload(wid)
     L, T, D, seed = run["solutions"], run["parallel_task"], run["converge_depth"], run["seed"]
     prob = Problem.from_spec(spec)
-    tile = None
-    if not a.no_tune:  # launch shape only; results are identical for every tile
+    tile = a.tile or None
+    if not a.no_tune and not tile:  # launch shape only; results are identical for every tile
         from pyhea_amd.autotune import tune_session
 
         tile = tune_session(prob, L, T, D, seed=seed, cache_path=None)["tile"]

@@ -8,8 +8,9 @@ produce the same bits):
 * ``grid`` -- persistent workgroups of a batched launch.
 
 ``tune_batch`` times one global iteration for each candidate on the actual
-problems and keeps the fastest; ``tune_session`` does the same for a single
-problem.  Choices are cached in a JSON file keyed by the workload signature
+problems (after one untimed iteration, which also captures the HIP graph) and
+keeps the fastest; ``tune_session`` does the same for a single problem,
+timing two iterations.  Choices are cached in a JSON file keyed by the workload signature
 (lattice sizes, species counts, shell sizes, population, parallel_task,
 converge_depth), so later runs reuse them without measuring.
 """
@@ -56,7 +57,7 @@ def _store(path, cache):
 
 
 def _tiles(T: int, S: int):
-    return [t for t in (64, 128, 192, 256) if S * t <= 1024 and t <= max(64, T)]
+    return [t for t in (64, 128, 192, 256) if t <= max(64, T)]
 
 
 def tune_batch(problems, solutions: int, parallel_task: int, converge_depth: int,
@@ -80,6 +81,7 @@ def tune_batch(problems, solutions: int, parallel_task: int, converge_depth: int
             if guide:
                 b.guide(**guide)
             b.init()
+            b.step(1)
             b.sync()
             t0 = time.perf_counter()
             b.step(1)
@@ -111,9 +113,10 @@ def tune_session(problem: Problem, solutions: int, parallel_task: int, converge_
         if guide:
             s.guide(**guide)
         s.init()
+        s.step(1)
         s.sync()
         t0 = time.perf_counter()
-        s.step(1)
+        s.step(2)
         s.sync()
         dt = time.perf_counter() - t0
         s.close()

@@ -864,11 +864,13 @@ int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, do
   return HEA_OK;
 }
 
-static int valid_tp(int tp, int S) { return tp >= 64 && tp <= 256 && tp % 64 == 0 && S * tp <= 1024; }
+static int valid_tp(int tp, int S) {
+  return tp >= 64 && tp <= HEA_MC_MAX_TP && tp % 64 == 0 && S * tp <= HEA_MC_MAX_THREADS;
+}
 
 int hea_session_tile(hea_session s, int32_t tp) {
   if (!s || !valid_tp(tp, s->p->hc.S))
-    return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256 with shells*tile <= 1024");
+    return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256");
   HEA_HIP(hipSetDevice(s->p->device));
   int rc;
   size_t lds;
@@ -1169,7 +1171,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   b->thr = thr;
   b->seed = seed;
   b->S = S;
-  b->kmax = kmax_need <= 6 ? 6 : 8;
+  b->kmax = kmax_need <= 5 ? 5 : kmax_need <= 6 ? 6 : 8;  // batch kernel widths
   b->tp = hea_mc_tp(task, S);
   b->threads = S * b->tp;
   b->sp_lds = !probs[0]->force_hbm_species;
@@ -1276,7 +1278,7 @@ static int batch_patch_args(hea_batch b, const std::function<void(HeaMcArgs&, in
 
 int hea_batch_tile(hea_batch b, int32_t tp) {
   if (!b || !valid_tp(tp, b->S))
-    return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256 with shells*tile <= 1024");
+    return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256");
   int maxN = 0;
   for (auto q : b->probs) maxN = std::max(maxN, q->hc.N);
   b->tp = tp;

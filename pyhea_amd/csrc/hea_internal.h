@@ -10,7 +10,8 @@
 #define HEA_MAX_K 8        // species per lattice handled by the kernels
 #define HEA_MAX_S 3        // neighbour shells (the reference builds 3)
 #define HEA_MAX_Z 31       // members per shell (5-bit packed counters)
-#define HEA_MC_THREADS 256  // threads of one local-search workgroup
+#define HEA_MC_MAX_TP 256   // proposals per pass (lanes per shell) at most
+#define HEA_MC_MAX_THREADS (HEA_MAX_S * HEA_MC_MAX_TP)  // local-search workgroup size at most
 #define HEA_MAX_TASK 2048  // proposals per round (parallel_task)
 #define HEA_MAX_SEG 160    // binade segments of one shell's R_s(k) table
 

@@ -59,7 +59,7 @@
 
 namespace {
 
-constexpr int kMaxWaves = 16;  // 1024 lanes
+constexpr int kMaxWaves = HEA_MC_MAX_THREADS / 64;
 constexpr double kU = 0x1p-53;  // unit roundoff of binary64
 constexpr int kMapWords = 1024;  // 32768-bit conflict map (guided mode)
 
@@ -69,7 +69,7 @@ __device__ __forceinline__ uint32_t map_bit(int i) { return ((uint32_t)i * 0x9e3
 
 struct McShared {
   int wcount[kMaxWaves];
-  double wmin[kMaxWaves];                // per-wave smallest screen value
+  double wmin[kMaxWaves];  // per-wave smallest screen value
   unsigned long long bmax[HEA_MAX_S];   // bits of the largest per-entry screen error bound
   unsigned long long ubmax[HEA_MAX_S];  // bits of the largest bound on |d'| (see refresh_entry)
   double beta;                          // candidate window half-width of this round
@@ -82,7 +82,7 @@ struct McShared {
   double pf, fest;
   int pfm;
   uint32_t next_pos;
-  int ncl;        // entries of the candidate list (a superset, see combine step)
+  int ncl;        // entries of the candidate list (a superset, see the combine step)
   unsigned gmax;  // guided mode: float bits of max_s Z_s |H_s| (gain scale)
 };
 
@@ -325,6 +325,38 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One step of a DPP reduction: v = min(v, v of the lane DPP control CTRL
+// names), in the rows ROWS enables; lanes of other rows keep v.  Every
+// control used here names a lane of the same wave, so no lane reads an
+// out-of-range source.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_min_step(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = (int)(uint32_t)b, hi = (int)(uint32_t)(b >> 32);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+  const double o = __longlong_as_double(
+      (long long)(((unsigned long long)(uint32_t)ohi << 32) | (uint32_t)olo));
+  return fmin(v, o);
+}
+
+// Smallest v over all 64 lanes (every lane active), returned to every lane.
+// row_ror:1,2,4,8 leave each 16-lane row's minimum in all its lanes;
+// row_bcast:15 folds rows 0 and 2 into rows 1 and 3, row_bcast:31 folds
+// row 1 into rows 2 and 3, so lane 63 holds the wave's minimum.
+__device__ __forceinline__ double wave_min(double v) {
+  v = dpp_min_step<0x121, 0xf>(v);
+  v = dpp_min_step<0x122, 0xf>(v);
+  v = dpp_min_step<0x124, 0xf>(v);
+  v = dpp_min_step<0x128, 0xf>(v);
+  v = dpp_min_step<0x142, 0xa>(v);
+  v = dpp_min_step<0x143, 0xc>(v);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Refreshes the tables of entry (s,i,j) from prev_gamma and records its
 // screen error bounds.
 //
@@ -535,7 +567,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   using PT = typename Pack<KMAX>::T;
   const HeaConst& P = *A.pc;
   const int nthr = blockDim.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: branches on it are scalar
   const int N = A.N, K = A.K, S = A.S;
   const int task = A.task, TP = A.tp;
   const int s_me = __builtin_amdgcn_readfirstlane(tid / TP);  // this wave's shell
@@ -594,6 +627,15 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   int acc = 0;
   long long rounds = 0, accepts = 0, screens = 0;
   bool hit_thr = false;
+  // Plain mode: the next round's first draw pass (stream positions pos + tid)
+  // is drawn by the waves that wait during the exact stage; wave 1 also draws
+  // wave 0's share, handed over through dq.  After an accept its validity
+  // ballot is published by the accept's barrier.
+  uint32_t nx1 = 0, nx2 = 0;
+  bool nxt = false;        // nx1, nx2 hold the pair at pos + tid
+  bool nvalid = false;     // its validity, when nbal_ok
+  bool nbal_ok = false;    // this wave's ballot of the pass is in wcount already
+  int2* dq = reinterpret_cast<int2*>(T.cmap);  // [64] (cmap is unused in plain mode)
 
   constexpr bool guided = GUIDED;
   // guided mode: hist[k] = packed neighbour species counts of site k, one
@@ -695,15 +737,29 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     }
     // ---- draw `task` valid proposals in stream order (accelerate.cpp:649-655)
     int found = guided ? task : 0;
+    bool first = true;
     while (found < task) {
       const uint32_t p = pos + tid;
       uint32_t s1, s2;
-      hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, p, (uint32_t)N, &s1, &s2);
-      const bool valid = sp[s1] != sp[s2];
-      const unsigned long long bal = __ballot(valid);
+      if (first && nxt) {
+        s1 = nx1;
+        s2 = nx2;
+      } else {
+        hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, p, (uint32_t)N, &s1, &s2);
+      }
+      bool valid;
+      unsigned long long bal;
+      if (first && nbal_ok) {
+        valid = nvalid;
+        bal = __ballot(valid);
+      } else {
+        valid = sp[s1] != sp[s2];
+        bal = __ballot(valid);
+        if (lane == 0) T.sh->wcount[wave] = __popcll(bal);
+        __syncthreads();
+      }
+      first = false;
       const int pre = __popcll(bal & ((1ull << lane) - 1ull));
-      if (lane == 0) T.sh->wcount[wave] = __popcll(bal);
-      __syncthreads();
       int off = 0, tot = 0;
 #pragma unroll
       for (int w = 0; w < kMaxWaves; ++w) {
@@ -719,6 +775,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       __syncthreads();
       if (found >= task) pos = T.sh->next_pos;
     }
+    nxt = false;
+    nbal_ok = false;
 
     // Guided mode accepts up to g_multi swaps per round: after an accept,
     // the proposals that touch none of this round's swapped sites or their
@@ -768,8 +826,10 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       }
     }
     __syncthreads();
-    // ---- X = sum_s w_s x_s and its minimum
+    // ---- X = sum_s w_s x_s, each wave's minimum and its candidate list
     double lmin = __builtin_inf();
+    double X1 = 0.0;  // this lane's X when the round fits one pass
+    bool has1 = false;
     for (int j = tid; j < task; j += nthr) {
       const int2 pr = T.prop[j];
       if (pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr))) {
@@ -777,28 +837,40 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         for (int s = 0; s < S; ++s) X = X + A.weight[s] * T.xs[s * task + j];
         T.xs[j] = X;
         lmin = fmin(lmin, X);
+        X1 = X;
+        has1 = true;
       }
     }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) lmin = fmin(lmin, __shfl_xor(lmin, m));
+    lmin = wave_min(lmin);
     if (lane == 0) T.sh->wmin[wave] = lmin;
     {
       // this wave's proposals within 2 beta of the wave's minimum: a superset
       // of the final candidates, since the workgroup's minimum is no larger
       const double wthr = lmin + 2.0 * T.sh->beta;
-      for (int j0 = 0; j0 < task; j0 += nthr) {
-        const int j = j0 + tid;
-        bool f = false;
-        if (j < task) {
-          const int2 pr = T.prop[j];
-          f = pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr)) && !(T.xs[j] > wthr);
-        }
+      if (task <= nthr) {  // one pass: lane tid holds proposal tid's X
+        const bool f = has1 && !(X1 > wthr);
         const unsigned long long bal = __ballot(f);
         if (bal) {
           int base = 0;
           if (lane == 0) base = atomicAdd(&T.sh->ncl, __popcll(bal));
           base = __builtin_amdgcn_readfirstlane(base);
-          if (f) T.cl[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+          if (f) T.cl[base + __popcll(bal & ((1ull << lane) - 1ull))] = tid;
+        }
+      } else {
+        for (int j0 = 0; j0 < task; j0 += nthr) {
+          const int j = j0 + tid;
+          bool f = false;
+          if (j < task) {
+            const int2 pr = T.prop[j];
+            f = pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr)) && !(T.xs[j] > wthr);
+          }
+          const unsigned long long bal = __ballot(f);
+          if (bal) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&T.sh->ncl, __popcll(bal));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (f) T.cl[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+          }
         }
       }
     }
@@ -815,11 +887,12 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // filter the waves' lists against the workgroup's window, compacting
       // in place (every entry is read before any of its chunk is written)
       const int nsup = T.sh->ncl;
-      int nc = 0;
+      int nc = 0, jfirst = -1;  // jfirst = cl[0] after the compaction
       for (int k0 = 0; k0 < nsup; k0 += 64) {
         const int j = (k0 + lane < nsup) ? T.cl[k0 + lane] : -1;
         const bool f = j >= 0 && !(T.xs[j] > thr);
         const unsigned long long bal = __ballot(f);
+        if (nc == 0 && bal) jfirst = __builtin_amdgcn_readlane(j, (int)__builtin_ctzll(bal));
         wave_sync();
         if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = j;
         nc += __popcll(bal);
@@ -835,6 +908,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       const double* grp = T.xw + g * K * XW;  // lane (c, s, y) at grp + y * XW
       unsigned long long bk = ~0ull;
       int bj = 0x7fffffff, bc = 0, wc0 = -1;
+      int ra = 0, rb = 0;  // window-decided accept: the winner's species (lanes < S*K)
       // When prev_fit = F(gamma), the window decides most rounds: every
       // proposal's err lies within beta of E + X (E common, X = 0 for the
       // current lattice), and 2 beta keeps sqrt's rounding out of the way.
@@ -858,32 +932,40 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         if (lane == 0) T.sh->res_kind = 0;
       } else {
       for (int c0 = 0; c0 < nc; c0 += cpc) {
-        const int j = (lane < cpc * GS && c0 + c < nc) ? T.cl[c0 + c] : -1;
+        int j = -1;
+        if (nc == 1)  // the usual case: the candidate came with the filter's ballot
+          j = (lane < GS) ? jfirst : -1;
+        else if (lane < cpc * GS && c0 + c < nc)
+          j = T.cl[c0 + c];
         int a = 0, b = 0;
+        double nA = 0.0, nB = 0.0;
         if (j >= 0) {
           const int2 pr = T.prop[j];
           a = sp[pr.x];
           b = sp[pr.y];
           const int w = s * task + j;
           const uint32_t cp = T.WC[w];
-          double nA, nB;
           exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w],
                               (int)(cp & 255u), (int)((cp >> 8) & 255u),
                               (int)((cp >> 16) & 255u), (int)(cp >> 24), nA, nB);
           slot[0] = nA;
           slot[1] = nB;
-          slot[2] = 1.0 - nA / (T.conc[a] * T.conc[x]);
-          slot[3] = 1.0 - nB / (T.conc[b] * T.conc[x]);
+          if (kind != 2) {  // alphas feed the terms, which a decided accept skips
+            slot[2] = 1.0 - nA / (T.conc[a] * T.conc[x]);
+            slot[3] = 1.0 - nB / (T.conc[b] * T.conc[x]);
+          }
         }
         wave_sync();
         if (kind == 2) {  // the one candidate is accepted: its new rows only
           if (j >= 0) {
-            T.G[T.idx(s, a, x)] = slot[0];
-            T.G[T.idx(s, x, a)] = slot[0];
-            T.G[T.idx(s, b, x)] = slot[1];
-            T.G[T.idx(s, x, b)] = slot[1];
+            T.G[T.idx(s, a, x)] = nA;
+            T.G[T.idx(s, x, a)] = nA;
+            T.G[T.idx(s, b, x)] = nB;
+            T.G[T.idx(s, x, b)] = nB;
           }
-          bj = T.cl[0];
+          bj = jfirst;
+          ra = a;
+          rb = b;
           break;
         }
         if (j >= 0) {  // terms of row x (accelerate.cpp:560-566)
@@ -957,12 +1039,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           T.sh->ubmax[lane] = 0ull;
         }
         if (lane == 0) {
-          const int2 pr = T.prop[bj];
           T.sh->gmax = 0u;
           T.sh->res_kind = 2;
           T.sh->res_j = bj;
-          T.sh->res_a = sp[pr.x];
-          T.sh->res_b = sp[pr.y];
+          T.sh->res_a = ra;
+          T.sh->res_b = rb;
           T.sh->pfm = 2;
           T.sh->fest = fest + xmin;
         }
@@ -1014,10 +1095,27 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       }
       }
       }
+    } else if constexpr (!GUIDED) {
+      hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pos + (uint32_t)tid, (uint32_t)N, &nx1,
+                      &nx2);
+      if (wave == 1) {
+        uint32_t u1, u2;
+        hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pos + (uint32_t)lane, (uint32_t)N,
+                        &u1, &u2);
+        dq[lane] = make_int2((int)u1, (int)u2);
+      }
     }
     __syncthreads();
     const int kind = T.sh->res_kind;
     if (sub == 0) ++rounds;
+    if (!GUIDED && nthr > 64) {
+      if (wave == 0) {
+        const int2 v = dq[lane];
+        nx1 = (uint32_t)v.x;
+        nx2 = (uint32_t)v.y;
+      }
+      nxt = true;
+    }
 
     // ---- accept iff strictly better (accelerate.cpp:678-689)
     if (kind != 0) {
@@ -1029,6 +1127,13 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       if (tid == 0) {
         sp[pr.x] = (uint8_t)b;
         sp[pr.y] = (uint8_t)a;
+      }
+      // species at the next pass's pair, loaded before the refresh so the
+      // loads overlap it (sp at pr.x, pr.y may be stale here: replaced below)
+      int q1 = 0, q2 = 0;
+      if (!GUIDED && nxt) {
+        q1 = sp[nx1];
+        q2 = sp[nx2];
       }
       if constexpr (GUIDED) {
         // neighbours of the swapped sites see a -> b (around pr.x) and
@@ -1054,6 +1159,14 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       for (int e = tid; e < S * K * K; e += nthr) {
         const int s = e / (K * K), i = (e / K) % K, j = e % K;
         refresh_entry<KMAX, GUIDED>(A, T, s, i, j);
+      }
+      if (!GUIDED && nxt) {  // validity of the next pass after this swap
+        const int v1 = (nx1 == (uint32_t)pr.x) ? b : (nx1 == (uint32_t)pr.y) ? a : q1;
+        const int v2 = (nx2 == (uint32_t)pr.x) ? b : (nx2 == (uint32_t)pr.y) ? a : q2;
+        nvalid = v1 != v2;
+        const unsigned long long nb = __ballot(nvalid);
+        if (lane == 0) T.sh->wcount[wave] = __popcll(nb);
+        nbal_ok = true;
       }
       __syncthreads();
       if (hit_thr && tid == 0 && A.first_hit) {
@@ -1092,7 +1205,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
 }
 
 template <int KMAX, bool SP_LDS, bool EXACT, bool SELF, bool GUIDED>
-__global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
+// At most 128 VGPRs (4 waves per SIMD), so two 6-wave workgroups share a CU.
+__global__ __launch_bounds__(HEA_MC_MAX_THREADS, 4) void k_local_mc(HeaMcArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   mc_lattice<KMAX, SP_LDS, EXACT, SELF, GUIDED>(A, blockIdx.x, smem);
 }
@@ -1102,7 +1216,7 @@ __global__ __launch_bounds__(1024) void k_local_mc(HeaMcArgs A) {
 // queue is empty.  B.probs[p] holds problem p's complete launch arguments
 // (its seed is the batch seed + p).
 template <int KMAX, bool SP_LDS, bool GUIDED>
-__global__ __launch_bounds__(1024) void k_local_mc_batch(HeaMcBatchArgs B) {
+__global__ __launch_bounds__(HEA_MC_MAX_THREADS, 4) void k_local_mc_batch(HeaMcBatchArgs B) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int item_sh;
   for (;;) {
@@ -1170,9 +1284,18 @@ const void* mc_batch_fn(bool sp_lds) {
                 : reinterpret_cast<const void*>(k_local_mc_batch<KMAX, false, GUIDED>);
 }
 
+// Batch kernels exist for table widths 5, 6 and 8 (guided: 5 and 6, its
+// packed neighbour histograms hold 6 species); width 5 keeps the tables of
+// 108000-site, 5-species problems small enough for their species to fit in
+// LDS.
 const void* mc_batch_fn_for(int kmax, bool sp_lds, bool guided) {
-  if (guided) return kmax <= 6 ? mc_batch_fn<6, true>(sp_lds) : nullptr;
-  return kmax <= 6 ? mc_batch_fn<6, false>(sp_lds) : mc_batch_fn<8, false>(sp_lds);
+  if (guided) {
+    if (kmax == 5) return mc_batch_fn<5, true>(sp_lds);
+    return kmax == 6 ? mc_batch_fn<6, true>(sp_lds) : nullptr;
+  }
+  if (kmax == 5) return mc_batch_fn<5, false>(sp_lds);
+  if (kmax == 6) return mc_batch_fn<6, false>(sp_lds);
+  return kmax == 8 ? mc_batch_fn<8, false>(sp_lds) : nullptr;
 }
 
 }  // namespace
@@ -1204,10 +1327,10 @@ int hea_mc_kmax(int K, bool self) {
 }
 
 // Proposals evaluated per pass (lanes per shell): a multiple of 64, at most
-// 256, with S * TP <= 1024 lanes.
+// HEA_MC_MAX_TP, so S * TP <= HEA_MC_MAX_THREADS lanes.
 int hea_mc_tp(int task, int S) {
   const int want = ((task + 63) / 64) * 64;
-  const int cap = std::min(256, (1024 / S) / 64 * 64);
+  const int cap = std::min(HEA_MC_MAX_TP, (HEA_MC_MAX_THREADS / S) / 64 * 64);
   return std::min(want, cap);
 }
 

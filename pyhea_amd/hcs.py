@@ -226,6 +226,12 @@ class Batch:
                                        C.byref(st)))
         return lat, fit, st.as_dict()
 
+    def best(self, p: int = 0) -> float:
+        """Lowest stored fitness of problem p (fitness only, no lattices)."""
+        fit = np.empty(self.solutions, dtype=np.float64)
+        N.check(N.lib().hea_batch_read(self._h, p, None, _ptr(fit, _dp), None))
+        return float(fit.min())
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             N.lib().hea_batch_destroy(self._h)
