@@ -185,8 +185,13 @@ def time_to_fitness(a):
         lat_f, fit, st = s.read() if mode != "guided-batched" else s.read(0)
         if mode == "plain":
             plain_final = (lat_f, fit)
+        # algorithmic bytes of the walked proposals (rounds x parallel_task;
+        # guided re-screens from stored walks read no neighbour rows)
+        walked_gbs = st["rounds"] * T * bytes_per_eval(spec.sum_z) / trace[-1][0] / 1e9
         modes[mode] = {"best": float(fit.min()), "iterations": len(trace), "wall_s": trace[-1][0],
                        "evals": st["evals"], "evals_per_s": st["evals"] / trace[-1][0],
+                       "achieved_gbs": round(walked_gbs, 1),
+                       "hbm_frac": round(walked_gbs / HBM_PEAK_GBS, 4),
                        "time_to_thr_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,
                        "trace": trace}
         s.close()

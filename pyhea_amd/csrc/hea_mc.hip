@@ -280,7 +280,7 @@ struct Lds {
   uint32_t* cmap;  // [kMapWords] guided mode: hashed sites touched by this round's accepts
   PT* WA;        // [S][task] walk results: histogram of a1's neighbours
   PT* WB;        // [S][task] histogram of a2's neighbours
-  uint32_t* WC;  // [S][task] c11 | c12 << 8 | c21 << 16 | c22 << 24
+  uint32_t* WC;  // [S][task] walk_pack(c11, c12, c21, c22, a, b)
   int* cl;       // [task] exact-stage candidate list
   McShared* sh;
   uint8_t* sp;   // [N] when resident
@@ -323,6 +323,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Walk record of one (proposal, shell): the adjacency counts c11, c12, c21,
+// c22 (each <= HEA_MAX_Z, 5 bits) and the species a, b of the two sites.
+struct WalkRec {
+  int c11, c12, c21, c22, a, b;
+};
+__device__ __forceinline__ uint32_t walk_pack(int c11, int c12, int c21, int c22, int a, int b) {
+  return (uint32_t)c11 | ((uint32_t)c12 << 5) | ((uint32_t)c21 << 10) | ((uint32_t)c22 << 15) |
+         ((uint32_t)a << 20) | ((uint32_t)b << 24);
+}
+__device__ __forceinline__ WalkRec walk_unpack(uint32_t w) {
+  return WalkRec{(int)(w & 31u),         (int)((w >> 5) & 31u), (int)((w >> 10) & 31u),
+                 (int)((w >> 15) & 31u), (int)((w >> 20) & 15u), (int)((w >> 24) & 15u)};
 }
 
 // One step of a DPP reduction: v = min(v, v of the lane DPP control CTRL
@@ -574,6 +588,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   const int s_me = __builtin_amdgcn_readfirstlane(tid / TP);  // this wave's shell
   const int jl = tid - s_me * TP;                             // proposal slot in a pass
   const int nq = (task + TP - 1) / TP;                        // passes per round
+  int s_beta = 0;
+  for (int t = 1; t < S; ++t)
+    if (A.Z[t] < A.Z[s_beta]) s_beta = t;
 
   Lds<KMAX> T;
   T.carve(smem, task, SP_LDS);
@@ -786,9 +803,10 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       for (int e = tid; e < kMapWords; e += nthr) T.cmap[e] = 0u;
     for (int sub = 0;; ++sub) {
     ++screens;
-    // window of the current tables and an empty candidate list; the screen's
-    // barrier publishes both
-    if (tid == 0) {
+    // window of the current tables and an empty candidate list, set by the
+    // first lane of the shell with the fewest neighbours (its waves finish the
+    // screen first); the screen's barrier publishes both
+    if (tid == s_beta * TP) {
       T.sh->beta = screen_beta<KMAX>(A, T);
       T.sh->ncl = 0;
     }
@@ -801,10 +819,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         if (GUIDED && sub > 0) {
           if (pr.x >= 0 && !touched(T, pr)) {  // walk stored by the first screen
             const int w = s_me * task + j;
-            const uint32_t cp = T.WC[w];
-            x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, sp[pr.x], sp[pr.y], T.WA[w],
-                                          T.WB[w], (int)(cp & 255u), (int)((cp >> 8) & 255u),
-                                          (int)((cp >> 16) & 255u), (int)(cp >> 24));
+            const WalkRec r = walk_unpack(T.WC[w]);
+            x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, r.a, r.b, T.WA[w], T.WB[w],
+                                          r.c11, r.c12, r.c21, r.c22);
           }
         } else if (pr.x >= 0) {
           const int a = sp[pr.x], b = sp[pr.y];
@@ -817,8 +834,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           const int w = s_me * task + j;
           T.WA[w] = MA;
           T.WB[w] = MB;
-          T.WC[w] = (uint32_t)c11 | ((uint32_t)c12 << 8) | ((uint32_t)c21 << 16) |
-                    ((uint32_t)c22 << 24);
+          T.WC[w] = walk_pack(c11, c12, c21, c22, a, b);
           x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, a, b, MA, MB, c11, c12, c21,
                                         c22);
         }
@@ -879,6 +895,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     // ---- exact stage (wave 0): candidates within the window, first strict
     // minimum of F (accelerate.cpp:663-674) and the accept test (:678)
     if (wave == 0) {
+      // the exact stage is this workgroup's critical path: issue priority over
+      // the waves of other workgroups on the CU until it ends
+      __builtin_amdgcn_s_setprio(2);
       double xmin = T.sh->wmin[0];
 #pragma unroll
       for (int w = 1; w < kMaxWaves; ++w) xmin = fmin(xmin, T.sh->wmin[w]);
@@ -940,14 +959,13 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         int a = 0, b = 0;
         double nA = 0.0, nB = 0.0;
         if (j >= 0) {
-          const int2 pr = T.prop[j];
-          a = sp[pr.x];
-          b = sp[pr.y];
+          const int2 pr = T.prop[j];  // only the rare replay path reads the sites
           const int w = s * task + j;
-          const uint32_t cp = T.WC[w];
-          exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w],
-                              (int)(cp & 255u), (int)((cp >> 8) & 255u),
-                              (int)((cp >> 16) & 255u), (int)(cp >> 24), nA, nB);
+          const WalkRec r = walk_unpack(T.WC[w]);
+          a = r.a;
+          b = r.b;
+          exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w], r.c11, r.c12,
+                              r.c21, r.c22, nA, nB);
           slot[0] = nA;
           slot[1] = nB;
           if (kind != 2) {  // alphas feed the terms, which a decided accept skips
@@ -1057,11 +1075,10 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         if (wc0 + cpc < nc) {  // a later chunk overwrote the winner's entries
           if (lane < GS) {
             const int w = s * task + bj;
-            const uint32_t cp = T.WC[w];
+            const WalkRec r = walk_unpack(T.WC[w]);
             double nA, nB;
-            exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w],
-                                (int)(cp & 255u), (int)((cp >> 8) & 255u),
-                                (int)((cp >> 16) & 255u), (int)(cp >> 24), nA, nB);
+            exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w], r.c11,
+                                r.c12, r.c21, r.c22, nA, nB);
             slot[0] = nA;
             slot[1] = nB;
           }
@@ -1095,6 +1112,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       }
       }
       }
+      __builtin_amdgcn_s_setprio(0);
     } else if constexpr (!GUIDED) {
       hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pos + (uint32_t)tid, (uint32_t)N, &nx1,
                       &nx2);
