@@ -476,6 +476,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
     a.delta[s] = c.delta[s];
     a.weight[s] = c.weight[s];
   }
+  hea_mc_split(a, p->hc.self_nbr != 0);
   HEA_HIP(hea_launch_local_mc(a, p->hc.K, sp_lds, p->hc.self_nbr != 0, lds, st));
   return HEA_OK;
 }
@@ -605,7 +606,9 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
       return fail(HEA_EUNSUPPORTED, "gamma sum has too many binade segments");
     }
   }
-  std::vector<int32_t> padded((size_t)N * c.stride, -1);
+  // 16 entries of slack after the last row: the walk of a split list may load
+  // (and mask) up to three 16-byte chunks past its part
+  std::vector<int32_t> padded((size_t)N * c.stride + 16, -1);
   {
     int o = 0;
     for (int s = 0; s < S; ++s) {
@@ -1173,7 +1176,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   b->S = S;
   b->kmax = kmax_need <= 5 ? 5 : kmax_need <= 6 ? 6 : 8;  // batch kernel widths
   b->tp = hea_mc_tp(task, S);
-  b->threads = S * b->tp;
+  b->threads = 0;  // the largest screen lane count of the problems (set with their args)
   b->sp_lds = !probs[0]->force_hbm_species;
   b->lds = hea_mc_lds_bytes(b->kmax, task, b->tp, maxN, b->sp_lds);
   if ((int64_t)b->lds > probs[0]->max_lds) {
@@ -1234,6 +1237,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
       a.task = task;
       a.depth = depth;
       a.tp = b->tp;
+      b->threads = std::max(b->threads, hea_mc_split(a, false));
       a.thr = thr;
       a.seed = seed + (uint64_t)q;
       a.phase = (uint32_t)(HEA_PHASE_LS1 + ph);
@@ -1282,7 +1286,7 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   int maxN = 0;
   for (auto q : b->probs) maxN = std::max(maxN, q->hc.N);
   b->tp = tp;
-  b->threads = b->S * tp;
+  b->threads = 0;
   b->sp_lds = !b->probs[0]->force_hbm_species;
   b->lds = hea_mc_lds_bytes(b->kmax, b->task, tp, maxN, b->sp_lds);
   if ((int64_t)b->lds > b->probs[0]->max_lds) {
@@ -1291,7 +1295,10 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   }
   HEA_HIP(hipSetDevice(b->probs[0]->device));
   HEA_HIP(hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds));
-  int rc = batch_patch_args(b, [tp](HeaMcArgs& a, int) { a.tp = tp; });
+  int rc = batch_patch_args(b, [tp, b](HeaMcArgs& a, int) {
+    a.tp = tp;
+    b->threads = std::max(b->threads, hea_mc_split(a, false));
+  });
   if (rc) return rc;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);

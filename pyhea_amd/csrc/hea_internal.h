@@ -11,7 +11,7 @@
 #define HEA_MAX_S 3        // neighbour shells (the reference builds 3)
 #define HEA_MAX_Z 31       // members per shell (5-bit packed counters)
 #define HEA_MC_MAX_TP 256   // proposals per pass (lanes per shell) at most
-#define HEA_MC_MAX_THREADS (HEA_MAX_S * HEA_MC_MAX_TP)  // local-search workgroup size at most
+#define HEA_MC_MAX_THREADS 1024  // local-search workgroup size at most
 #define HEA_MAX_TASK 2048  // proposals per round (parallel_task)
 #define HEA_MAX_SEG 160    // binade segments of one shell's R_s(k) table
 
@@ -50,7 +50,11 @@ struct HeaMcArgs {
   const uint32_t* iter_ptr; // stream iteration (device memory, so graphs can replay)
   unsigned long long* first_hit;  // wall clock of the first fitness <= thr (atomicMin)
   uint64_t seed;
-  int32_t L, task, depth, tp;   // tp: proposals per pass (lanes per shell)
+  int32_t L, task, depth, tp;   // tp: proposal slots per pass
+  // screen lanes (hea_mc_split): shell s's group starts at lane gbase[s] and
+  // walks its list in 2^glog[s] parts; gbase[S] = lanes of the workgroup
+  int32_t gbase[HEA_MAX_S + 1];
+  int32_t glog[HEA_MAX_S];
   uint32_t phase;
   double thr;
   // guided proposals (hist == nullptr: plain keyed stream)
@@ -85,6 +89,7 @@ __device__ __forceinline__ double hea_r_of_count(const HeaConst& P, int s, int64
 size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds);
 int hea_mc_kmax(int K, bool self);
 int hea_mc_tp(int task, int S);
+int hea_mc_split(HeaMcArgs& a, bool self);
 hipError_t hea_prepare_local_mc(int K, bool sp_lds, bool self, size_t lds);
 hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self, size_t lds,
                                hipStream_t st);

@@ -59,7 +59,7 @@
 
 namespace {
 
-constexpr int kMaxWaves = HEA_MC_MAX_THREADS / 64;
+constexpr int kMaxWaves = HEA_MC_MAX_THREADS / 64;  // 16
 constexpr double kU = 0x1p-53;  // unit roundoff of binary64
 constexpr int kMapWords = 1024;  // 32768-bit conflict map (guided mode)
 
@@ -67,7 +67,7 @@ constexpr int kMapWords = 1024;  // 32768-bit conflict map (guided mode)
 // the map more conservative).
 __device__ __forceinline__ uint32_t map_bit(int i) { return ((uint32_t)i * 0x9e3779b1u) >> 17; }
 
-struct McShared {
+struct McShared {  // wcount and wmin are read as 16-byte vectors
   int wcount[kMaxWaves];
   double wmin[kMaxWaves];  // per-wave smallest screen value
   unsigned long long bmax[HEA_MAX_S];   // bits of the largest per-entry screen error bound
@@ -147,9 +147,12 @@ __device__ __forceinline__ double replay_ab(double v, double delta, const int32_
 // Species histograms of the shell-s neighbour lists of a1 and a2 (MA, MB:
 // += 1 << (W * species)) and the adjacency counts cXY (occurrences of site Y
 // in X's list).  Lists are padded to a multiple of 4 entries with -1.  NC =
-// ceil(z/4) 16-byte chunks per list; chunks C0..C0+NG-1 of both lists are
-// loaded before any is used, so their loads are in flight together.
-template <int KMAX, bool SELF, int C0, int NG, bool LAST>
+// 16-byte chunks of both lists; chunks C0..C0+NG-1 are loaded before any is
+// used, so their loads are in flight together.  Entries at or beyond z are
+// masked: a whole list ends inside its last chunk (padding -1, which matches
+// no site); a part of a split list (PART) may end anywhere, and its loads
+// past the end read in-bounds entries of the next shell or row.
+template <int KMAX, bool SELF, bool PART, int C0, int NG, bool LAST>
 __device__ __forceinline__ void walk_group(const int32_t* __restrict__ r1,
                                            const int32_t* __restrict__ r2, int z,
                                            const uint8_t* __restrict__ sp, int a1, int a2,
@@ -168,16 +171,17 @@ __device__ __forceinline__ void walk_group(const int32_t* __restrict__ r1,
     const int nv[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const bool ok = !(LAST && c == NG - 1) || (4 * (C0 + c) + t < z);  // uniform
+      const bool ok = PART ? (4 * (C0 + c) + t < z)
+                           : (!(LAST && c == NG - 1) || (4 * (C0 + c) + t < z));
       const typename PK::T fa = PK::one((int)sp[ok ? nu[t] : a1]);
       const typename PK::T fb = PK::one((int)sp[ok ? nv[t] : a2]);
       ma += ok ? fa : (typename PK::T)0;
       mb += ok ? fb : (typename PK::T)0;
-      x12 += (nu[t] == a2);  // padding (-1) never matches
-      x21 += (nv[t] == a1);
+      x12 += (!PART || ok) && (nu[t] == a2);
+      x21 += (!PART || ok) && (nv[t] == a1);
       if (SELF) {
-        x11 += (nu[t] == a1);
-        x22 += (nv[t] == a2);
+        x11 += (!PART || ok) && (nu[t] == a1);
+        x22 += (!PART || ok) && (nv[t] == a2);
       }
     }
     // one chunk's LDS reads at a time (the group's row loads are in flight)
@@ -185,9 +189,9 @@ __device__ __forceinline__ void walk_group(const int32_t* __restrict__ r1,
   }
 }
 
-// Both lists of one shell in groups of at most 4 chunks (registers bound the
-// loads in flight).
-template <int KMAX, bool SELF, int NC>
+// NC chunks of both lists (z entries valid) in groups of at most 4 chunks
+// (registers bound the loads in flight).
+template <int KMAX, bool SELF, bool PART, int NC>
 __device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
                                           const int32_t* __restrict__ r2, int z,
                                           const uint8_t* __restrict__ sp, int a1, int a2,
@@ -196,10 +200,11 @@ __device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
   typename Pack<KMAX>::T ma = 0, mb = 0;
   int x11 = 0, x12 = 0, x21 = 0, x22 = 0;
   if constexpr (NC <= 4) {
-    walk_group<KMAX, SELF, 0, NC, true>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
+    walk_group<KMAX, SELF, PART, 0, NC, true>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
   } else {
-    walk_group<KMAX, SELF, 0, 4, false>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
-    walk_group<KMAX, SELF, 4, NC - 4, true>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
+    walk_group<KMAX, SELF, PART, 0, 4, false>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21, x22);
+    walk_group<KMAX, SELF, PART, 4, NC - 4, true>(r1, r2, z, sp, a1, a2, ma, mb, x11, x12, x21,
+                                                  x22);
   }
   MA = ma;
   MB = mb;
@@ -209,15 +214,15 @@ __device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
   c22 = x22;
 }
 
-template <int KMAX, bool SELF>
+template <int KMAX, bool SELF, bool PART>
 __device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
-                                          const int32_t* __restrict__ r2, int z,
+                                          const int32_t* __restrict__ r2, int z, int nch,
                                           const uint8_t* __restrict__ sp, int a1, int a2,
                                           typename Pack<KMAX>::T& MA, typename Pack<KMAX>::T& MB,
                                           int& c11, int& c12, int& c21, int& c22) {
-  switch ((z + 3) >> 2) {  // uniform across the wave (one shell per wave)
+  switch (nch) {  // uniform across the wave (one shell and part size per wave)
 #define HEA_WALK(n) \
-  case n: walk_pair<KMAX, SELF, n>(r1, r2, z, sp, a1, a2, MA, MB, c11, c12, c21, c22); break;
+  case n: walk_pair<KMAX, SELF, PART, n>(r1, r2, z, sp, a1, a2, MA, MB, c11, c12, c21, c22); break;
     HEA_WALK(1) HEA_WALK(2) HEA_WALK(3) HEA_WALK(4) HEA_WALK(5) HEA_WALK(6) HEA_WALK(7) HEA_WALK(8)
 #undef HEA_WALK
     default: break;
@@ -323,6 +328,19 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum of v over the 2^lg adjacent lanes that share a proposal (lg <= 2, the
+// same for the whole wave): DPP quad permutations, every source in the quad.
+__device__ __forceinline__ uint32_t part_sum(uint32_t v, int lg) {
+  if (lg > 0) v += (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xf, 0xf, false);
+  if (lg > 1) v += (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xf, 0xf, false);
+  return v;
+}
+// Packed histograms: fields never exceed HEA_MAX_Z, so sums carry into no
+// neighbouring field and the two halves of a 64-bit pack add separately.
+__device__ __forceinline__ uint64_t part_sum(uint64_t v, int lg) {
+  return (uint64_t)part_sum((uint32_t)v, lg) | ((uint64_t)part_sum((uint32_t)(v >> 32), lg) << 32);
 }
 
 // Walk record of one (proposal, shell): the adjacency counts c11, c12, c21,
@@ -581,16 +599,31 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   using PT = typename Pack<KMAX>::T;
   const HeaConst& P = *A.pc;
   const int nthr = blockDim.x;
+  const int nwaves = (nthr + 63) >> 6;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: branches on it are scalar
   const int N = A.N, K = A.K, S = A.S;
   const int task = A.task, TP = A.tp;
-  const int s_me = __builtin_amdgcn_readfirstlane(tid / TP);  // this wave's shell
-  const int jl = tid - s_me * TP;                             // proposal slot in a pass
-  const int nq = (task + TP - 1) / TP;                        // passes per round
-  int s_beta = 0;
+  // Screen lanes: shell s's group starts at lane gbase[s]; its list is walked
+  // in 2^glog[s] parts by adjacent lanes sharing a proposal slot.  Groups are
+  // whole waves, so every wave walks one shell with one part size.
+  int s_grp = 0;
   for (int t = 1; t < S; ++t)
+    if (tid >= A.gbase[t]) s_grp = t;
+  const int s_me = __builtin_amdgcn_readfirstlane(s_grp);  // this wave's shell
+  const bool scr = tid < A.gbase[S];                       // a screen lane (uniform per wave)
+  const int plg = A.glog[s_me];
+  const int jl = (tid - A.gbase[s_me]) >> plg;          // proposal slot in a pass
+  const int part = (tid - A.gbase[s_me]) & ((1 << plg) - 1);
+  const int pch = (((A.Z[s_me] + 3) >> 2) + (1 << plg) - 1) >> plg;  // chunks per part
+  const int pz = min(max(A.Z[s_me] - 4 * part * pch, 0), 4 * pch);  // entries of this part
+  const int poff = A.off[s_me] + 4 * part * pch;                     // its first column
+  const int nq = (task + TP - 1) / TP;                        // passes per round
+  int s_beta = 0, s_long = 0;  // shells with the fewest and the most neighbours
+  for (int t = 1; t < S; ++t) {
     if (A.Z[t] < A.Z[s_beta]) s_beta = t;
+    if (A.Z[t] > A.Z[s_long]) s_long = t;
+  }
 
   Lds<KMAX> T;
   T.carve(smem, task, SP_LDS);
@@ -779,10 +812,15 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       const int pre = __popcll(bal & ((1ull << lane) - 1ull));
       int off = 0, tot = 0;
 #pragma unroll
-      for (int w = 0; w < kMaxWaves; ++w) {
-        const int c = T.sh->wcount[w];
-        off += (w < wave) ? c : 0;
-        tot += c;
+      for (int w4 = 0; w4 < kMaxWaves / 4; ++w4) {
+        if (4 * w4 >= nwaves) break;  // slots of absent waves hold 0
+        const int4 c4 = reinterpret_cast<const int4*>(T.sh->wcount)[w4];
+        const int cw[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          off += (4 * w4 + k < wave) ? cw[k] : 0;
+          tot += cw[k];
+        }
       }
       const int rank = found + off + pre;
       if (valid && rank < task) T.prop[rank] = make_int2((int)s1, (int)s2);
@@ -806,31 +844,50 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     // window of the current tables and an empty candidate list, set by the
     // first lane of the shell with the fewest neighbours (its waves finish the
     // screen first); the screen's barrier publishes both
-    if (tid == s_beta * TP) {
+    if (tid == A.gbase[s_beta]) {
       T.sh->beta = screen_beta<KMAX>(A, T);
       T.sh->ncl = 0;
     }
     // ---- screen: walk both neighbour lists of every proposal, keep the walk
-    for (int q = 0; q < nq; ++q) {
+    // (the waves of the longest shell finish last: they issue first)
+    if (s_me == s_long) __builtin_amdgcn_s_setprio(1);
+    for (int q = 0; scr && q < nq; ++q) {
       const int j = jl + q * TP;
-      if (j < task) {
-        const int2 pr = T.prop[j];
-        double x = 0.0;
-        if (GUIDED && sub > 0) {
-          if (pr.x >= 0 && !touched(T, pr)) {  // walk stored by the first screen
-            const int w = s_me * task + j;
-            const WalkRec r = walk_unpack(T.WC[w]);
-            x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, r.a, r.b, T.WA[w], T.WB[w],
-                                          r.c11, r.c12, r.c21, r.c22);
-          }
-        } else if (pr.x >= 0) {
-          const int a = sp[pr.x], b = sp[pr.y];
-          const int32_t* r1 = A.nbr + (size_t)pr.x * A.stride + A.off[s_me];
-          const int32_t* r2 = A.nbr + (size_t)pr.y * A.stride + A.off[s_me];
-          const int z = A.Z[s_me];
-          PT MA = 0, MB = 0;
-          int c11 = 0, c12 = 0, c22 = 0, c21 = 0;
-          walk_pair<KMAX, SELF>(r1, r2, z, sp, pr.x, pr.y, MA, MB, c11, c12, c21, c22);
+      const int2 pr = (j < task) ? T.prop[j] : make_int2(-1, -1);
+      double x = 0.0;
+      if (GUIDED && sub > 0) {
+        if (part == 0 && pr.x >= 0 && !touched(T, pr)) {  // walk stored by the first screen
+          const int w = s_me * task + j;
+          const WalkRec r = walk_unpack(T.WC[w]);
+          x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, r.a, r.b, T.WA[w], T.WB[w],
+                                        r.c11, r.c12, r.c21, r.c22);
+        }
+      } else {
+        PT MA = 0, MB = 0;
+        int c11 = 0, c12 = 0, c22 = 0, c21 = 0, a = 0, b = 0;
+        if (pr.x >= 0) {
+          a = sp[pr.x];  // before the walk, so these loads overlap the row loads
+          b = sp[pr.y];
+          const int32_t* r1 = A.nbr + (size_t)pr.x * A.stride + poff;
+          const int32_t* r2 = A.nbr + (size_t)pr.y * A.stride + poff;
+          if (SELF || plg == 0)  // the whole list (lists of cells with self neighbours are never split)
+            walk_pair<KMAX, SELF, false>(r1, r2, pz, pch, sp, pr.x, pr.y, MA, MB, c11, c12, c21,
+                                         c22);
+          else
+            walk_pair<KMAX, SELF, true>(r1, r2, pz, pch, sp, pr.x, pr.y, MA, MB, c11, c12, c21,
+                                        c22);
+        }
+        if (plg > 0) {  // the parts' sums (all lanes of the wave take part)
+          MA = part_sum(MA, plg);
+          MB = part_sum(MB, plg);
+          const uint32_t cc = part_sum((uint32_t)c11 | ((uint32_t)c12 << 8) |
+                                           ((uint32_t)c21 << 16) | ((uint32_t)c22 << 24), plg);
+          c11 = (int)(cc & 255u);
+          c12 = (int)((cc >> 8) & 255u);
+          c21 = (int)((cc >> 16) & 255u);
+          c22 = (int)(cc >> 24);
+        }
+        if (part == 0 && pr.x >= 0) {
           const int w = s_me * task + j;
           T.WA[w] = MA;
           T.WB[w] = MB;
@@ -838,9 +895,10 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, a, b, MA, MB, c11, c12, c21,
                                         c22);
         }
-        T.xs[s_me * task + j] = x;
       }
+      if (part == 0 && j < task) T.xs[s_me * task + j] = x;
     }
+    if (s_me == s_long) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     // ---- X = sum_s w_s x_s, each wave's minimum and its candidate list
     double lmin = __builtin_inf();
@@ -898,9 +956,13 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // the exact stage is this workgroup's critical path: issue priority over
       // the waves of other workgroups on the CU until it ends
       __builtin_amdgcn_s_setprio(2);
-      double xmin = T.sh->wmin[0];
+      double xmin = __builtin_inf();
 #pragma unroll
-      for (int w = 1; w < kMaxWaves; ++w) xmin = fmin(xmin, T.sh->wmin[w]);
+      for (int w2 = 0; w2 < kMaxWaves / 2; ++w2) {
+        if (2 * w2 >= nwaves) break;  // slots of absent waves hold inf
+        const double2 m2 = reinterpret_cast<const double2*>(T.sh->wmin)[w2];
+        xmin = fmin(xmin, fmin(m2.x, m2.y));
+      }
       double thr = xmin + 2.0 * T.sh->beta;
       if (!(thr < __builtin_inf())) thr = __builtin_inf();
       // filter the waves' lists against the workgroup's window, compacting
@@ -1223,7 +1285,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
 }
 
 template <int KMAX, bool SP_LDS, bool EXACT, bool SELF, bool GUIDED>
-// At most 128 VGPRs (4 waves per SIMD), so two 6-wave workgroups share a CU.
+// At most 128 VGPRs (4 waves per SIMD): two 6- or 8-wave workgroups share a CU.
 __global__ __launch_bounds__(HEA_MC_MAX_THREADS, 4) void k_local_mc(HeaMcArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   mc_lattice<KMAX, SP_LDS, EXACT, SELF, GUIDED>(A, blockIdx.x, smem);
@@ -1338,6 +1400,42 @@ hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_
   return hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, st);
 }
 
+// Screen lanes of a launch with a.tp proposal slots per pass: shell s gets
+// 2^glog[s] lanes per slot, each walking one part of its neighbour list.  A
+// shell is split only when its list has at least twice the chunks (4 entries
+// each) of the next-longest list, into parts no shorter than that list: the
+// long shell then stops keeping the others waiting, and shorter gains do not
+// pay for the extra lanes (measured: FCC 12/6/24 gains, BCC 8/6/12 loses).
+// Tiny cells whose sites list themselves (self) keep whole lists.  Returns
+// the lanes.
+int hea_mc_split(HeaMcArgs& a, bool self) {
+  int nch[HEA_MAX_S], lg[HEA_MAX_S] = {0, 0, 0};
+  for (int s = 0; s < a.S; ++s) nch[s] = std::max(1, (a.Z[s] + 3) >> 2);
+  for (int s = 0; s < a.S && !self; ++s) {
+    int next = 0;  // longest other list
+    for (int t = 0; t < a.S; ++t)
+      if (t != s) next = std::max(next, nch[t]);
+    while (next > 0 && lg[s] < 2 && nch[s] >= (2 * next) << lg[s]) ++lg[s];
+  }
+  for (;;) {  // fit the workgroup: halve the widest split until it does
+    int lanes = 0, wide = 0;
+    for (int s = 0; s < a.S; ++s) {
+      lanes += a.tp << lg[s];
+      if (lg[s] > lg[wide]) wide = s;
+    }
+    if (lanes <= HEA_MC_MAX_THREADS || lg[wide] == 0) break;
+    --lg[wide];
+  }
+  int g = 0;
+  for (int s = 0; s < HEA_MAX_S; ++s) {
+    a.glog[s] = s < a.S ? lg[s] : 0;
+    a.gbase[s] = g;
+    if (s < a.S) g += a.tp << lg[s];
+  }
+  for (int s = a.S; s <= HEA_MAX_S; ++s) a.gbase[s] = g;
+  return g;
+}
+
 // Table width of the kernel chosen for K species (see mc_fn_for).
 int hea_mc_kmax(int K, bool self) {
   if (self || K == 7) return 8;
@@ -1382,5 +1480,5 @@ hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self
   const void* fn = mc_fn_for(K, sp_lds, true, self, a.hist != nullptr);
   if (!fn) return hipErrorInvalidValue;
   void* args[] = {const_cast<HeaMcArgs*>(&a)};
-  return hipLaunchKernel(fn, dim3(a.L), dim3(a.S * a.tp), args, lds, st);
+  return hipLaunchKernel(fn, dim3(a.L), dim3(a.gbase[a.S]), args, lds, st);
 }
