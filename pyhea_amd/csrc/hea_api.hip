@@ -264,7 +264,7 @@ struct DevBuf {
 
 // Guided-proposal settings of a local-search launch (gain == nullptr: plain).
 struct GuideCfg {
-  uint32_t* hist = nullptr;  // [L][N][S] packed neighbour species counts
+  unsigned long long* hist = nullptr;  // [L][N] per-site neighbour species counts
   int cand = 0;
   double tau = 0.0;
   int multi = 1;
@@ -305,7 +305,7 @@ struct hea_session_s {
   DevBuf<uint32_t> iter;
   DevBuf<unsigned long long> clk;  // [0] start, [1] first fitness <= thr
   int iters_done = 0;
-  DevBuf<uint32_t> hist;  // guided mode
+  DevBuf<unsigned long long> hist;  // guided mode, [L][N]
   GuideCfg guide;
   int tp = 0;             // proposals per pass (0: default)
 };
@@ -896,6 +896,9 @@ static int check_guide(hea_problem p, int candidates, double tau, int accepts) {
     return fail(HEA_EINVAL, "guide needs candidates >= 1, tau > 0, accepts >= 1");
   if (p->hc.K > 6) return fail(HEA_EUNSUPPORTED, "guided proposals support up to 6 species");
   if (p->hc.self_nbr) return fail(HEA_EUNSUPPORTED, "guided proposals need cells without self neighbours");
+  const HeaHistLayout h = hea_hist_layout(p->hc.S, p->hc.Z);
+  if ((p->hc.K - 1) * h.W > 64)
+    return fail(HEA_EUNSUPPORTED, "guided proposals need (K-1) * sum_s bitwidth(Z_s) <= 64");
   return HEA_OK;
 }
 
@@ -904,7 +907,7 @@ int hea_session_guide(hea_session s, int32_t candidates, double tau, int32_t acc
   int rc = check_guide(s->p, candidates, tau, accepts);
   if (rc) return rc;
   HEA_HIP(hipSetDevice(s->p->device));
-  const size_t n = (size_t)s->L * s->p->hc.N * 4;
+  const size_t n = (size_t)s->L * s->p->hc.N;
   if (s->hist.n < n) HEA_HIP(s->hist.alloc(n));
   size_t lds;
   bool sp;
@@ -1091,7 +1094,7 @@ struct hea_batch_s {
   DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
   DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
   DevBuf<int2> items;
-  DevBuf<uint32_t> hist;           // guided mode, [sum_p L N_p 4]
+  DevBuf<unsigned long long> hist;  // guided mode, [sum_p L N_p]
   GuideCfg guide;
 };
 
@@ -1320,10 +1323,10 @@ int hea_batch_guide(hea_batch b, int32_t candidates, double tau, int32_t accepts
   std::vector<size_t> off(P);
   for (int q = 0; q < P; ++q) {
     off[q] = n;
-    n += (size_t)b->L * b->probs[q]->hc.N * 4;
+    n += (size_t)b->L * b->probs[q]->hc.N;
   }
   if (b->hist.n < n) HEA_HIP(b->hist.alloc(n));
-  uint32_t* base = b->hist.p;
+  unsigned long long* base = b->hist.p;
   rc = batch_patch_args(b, [&](HeaMcArgs& a, int q) {
     a.hist = base + off[q];
     a.g_cand = candidates;

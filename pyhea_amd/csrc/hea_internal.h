@@ -58,7 +58,7 @@ struct HeaMcArgs {
   uint32_t phase;
   double thr;
   // guided proposals (hist == nullptr: plain keyed stream)
-  uint32_t* hist;               // [L][N][4] packed neighbour species counts per shell
+  unsigned long long* hist;     // [L][N] per-site neighbour species counts (HeaHistLayout)
   int32_t g_cand;               // candidate pairs per proposal
   double g_tau;                 // softmax temperature in units of the gain scale
   int32_t g_multi;              // accepts per round at most
@@ -77,6 +77,27 @@ struct HeaMcBatchArgs {
   int32_t n_items;
   uint32_t* queue;  // zeroed before each launch
 };
+
+// Guided mode's per-site record: the species counts of a site's neighbours,
+// all shells in one 64-bit word.  The count of species x < K-1 in shell s is
+// the field at bit x*W + o[s] of width w[s] (the bit width of Z_s; W = sum of
+// the widths); species K-1's count is Z_s minus the others.  Usable when
+// (K-1)*W <= 64 (FCC 12/6/24 needs 12 bits per species, BCC 8/6/12 11).
+struct HeaHistLayout {
+  int w[HEA_MAX_S], o[HEA_MAX_S], W;
+};
+__host__ __device__ inline HeaHistLayout hea_hist_layout(int S, const int32_t* Z) {
+  HeaHistLayout h;
+  h.W = 0;
+  for (int s = 0; s < HEA_MAX_S; ++s) {
+    int b = 0;
+    while (s < S && (1 << b) <= Z[s]) ++b;
+    h.w[s] = b;
+    h.o[s] = h.W;
+    h.W += b;
+  }
+  return h;
+}
 
 // R_s(k) from the binade segments (exact: the host verified every k).
 __device__ __forceinline__ double hea_r_of_count(const HeaConst& P, int s, int64_t k) {

@@ -688,18 +688,22 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   int2* dq = reinterpret_cast<int2*>(T.cmap);  // [64] (cmap is unused in plain mode)
 
   constexpr bool guided = GUIDED;
-  // guided mode: hist[k] = packed neighbour species counts of site k, one
-  // 32-bit word per shell (5-bit fields), 16 bytes per site
-  uint32_t* hist = guided ? A.hist + (size_t)l * N * 4 : nullptr;
+  // guided mode: hist[k] = neighbour species counts of site k (HeaHistLayout:
+  // 8 bytes per site, so the records of many lattices stay cache-resident)
+  unsigned long long* hist = guided ? A.hist + (size_t)l * N : nullptr;
+  const HeaHistLayout hl = hea_hist_layout(S, A.Z);
   if constexpr (GUIDED) {
     for (int k = tid; k < N; k += nthr) {
       const int32_t* row = A.nbr + (size_t)k * A.stride;
-      uint32_t m[4] = {0u, 0u, 0u, 0u};
+      unsigned long long m = 0ull;
       for (int s = 0; s < S; ++s)
-        for (int q = 0; q < A.Z[s]; ++q) m[s] += 1u << (5 * sp[row[A.off[s] + q]]);
-      *reinterpret_cast<uint4*>(hist + (size_t)k * 4) = make_uint4(m[0], m[1], m[2], m[3]);
+        for (int q = 0; q < A.Z[s]; ++q) {
+          const int c = sp[row[A.off[s] + q]];
+          if (c < K - 1) m += 1ull << (c * hl.W + hl.o[s]);
+        }
+      hist[k] = m;
     }
-    __syncthreads();  // every site's histogram before the first candidate reads
+    __syncthreads();  // every site's record before the first candidate reads
   }
   // guided mode: candidate stream key of this lattice and phase
   const uint32_t gk = hea_mix32(hea_mix32(hea_mix32((uint32_t)A.seed ^ 0x6a09e667u) ^
@@ -728,34 +732,36 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           sb[q] = sp[s2[q]];
           live[q] = w < items && sa[q] != sb[q];
         }
-        unsigned long long hv[4][4];
+        unsigned long long hv[4][2];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          // every access to a lattice's histograms comes from its own
+          // every access to a lattice's records comes from its own
           // workgroup: workgroup-scope atomics, loads and barriers order them
-          const unsigned long long* p1 = reinterpret_cast<const unsigned long long*>(hist + (size_t)s1[q] * 4);
-          const unsigned long long* p2 = reinterpret_cast<const unsigned long long*>(hist + (size_t)s2[q] * 4);
-          hv[q][0] = __hip_atomic_load(p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          hv[q][1] = __hip_atomic_load(p1 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          hv[q][2] = __hip_atomic_load(p2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          hv[q][3] = __hip_atomic_load(p2 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          hv[q][0] = __hip_atomic_load(hist + s1[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          hv[q][1] = __hip_atomic_load(hist + s2[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int a = sa[q], b = sb[q];
-          const uint32_t hh1[3] = {(uint32_t)hv[q][0], (uint32_t)(hv[q][0] >> 32), (uint32_t)hv[q][1]};
-          const uint32_t hh2[3] = {(uint32_t)hv[q][2], (uint32_t)(hv[q][2] >> 32), (uint32_t)hv[q][3]};
           // first-order change of F^2: sum_s sum_d (H_s[b][d] - H_s[a][d]) (cnt1 - cnt2);
-          // absent shells and species have zero rows and zero counts
+          // absent shells and species have zero rows; the last species'
+          // count difference is minus the others' (both lists hold Z_s)
           float d = 0.0f;
 #pragma unroll
           for (int sh = 0; sh < HEA_MAX_S; ++sh) {
             const float* Ha = T.H + T.hidx(sh, a, 0);
             const float* Hb = T.H + T.hidx(sh, b, 0);
+            const unsigned long long fm = (1ull << hl.w[sh]) - 1ull;
+            int rest = 0;
 #pragma unroll
-            for (int x = 0; x < KMAX; ++x)
-              d += (Hb[x] - Ha[x]) * (float)((int)((hh1[sh] >> (5 * x)) & 31u) -
-                                             (int)((hh2[sh] >> (5 * x)) & 31u));
+            for (int x = 0; x < KMAX - 1; ++x) {
+              const int sft = min(x * hl.W + hl.o[sh], 63);  // in range; x >= K-1 is masked
+              const int dc = (x < K - 1) ? (int)((hv[q][0] >> sft) & fm) - (int)((hv[q][1] >> sft) & fm) : 0;
+              rest -= dc;
+              d += (Hb[x] - Ha[x]) * (float)dc;
+            }
+            const float* Hl = Hb + (K - 1);  // species K-1
+            d += (*Hl - Ha[K - 1]) * (float)rest;
           }
           const float U = ((float)(h3[q] >> 8) + 0.5f) * 0x1p-24f;
           const float sc = -d * inv - __logf(-__logf(U));
@@ -1225,9 +1231,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           int q = t < zs ? t : t - zs, s = 0;
           while (q >= A.Z[s]) q -= A.Z[s++];
           const int nb = A.nbr[(size_t)site * A.stride + A.off[s] + q];
-          const uint32_t dv = t < zs ? (1u << (5 * b)) - (1u << (5 * a)) : (1u << (5 * a)) - (1u << (5 * b));
-          __hip_atomic_fetch_add(hist + (size_t)nb * 4 + s, dv, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          // the last species has no field (its count is implied)
+          const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + hl.o[s]) : 0ull;
+          const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + hl.o[s]) : 0ull;
+          const unsigned long long dv = t < zs ? fb - fa : fa - fb;  // modulo 2^64
+          __hip_atomic_fetch_add(hist + nb, dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           const uint32_t mb = map_bit(nb);
           atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
         }
