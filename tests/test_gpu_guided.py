@@ -62,3 +62,20 @@ def test_guided_batch_matches_guided_sessions(gpu):
         ls, fs, _ = s.read()
         lb, fb, _ = b.read(q)
         assert np.array_equal(lb, ls) and np.array_equal(fb, fs)
+
+
+def test_guide_rejects_records_wider_than_64_bits(gpu):
+    """A site's neighbour counts must fit one 64-bit record: each species but
+    the last takes a field as wide as its shell needs, here (6-1) x (5+5+4)
+    = 70 bits for shells of 30, 30 and 8 neighbours (a circulant graph on
+    200 sites, offsets +-1..15, +-16..30, +-31..34: symmetric, no self)."""
+    N, K = 200, 6
+    offs = [list(range(1, 16)), list(range(16, 31)), list(range(31, 35))]
+    cols = [d for sh in offs for d in (*sh, *[-x for x in sh])]
+    nbr = (np.arange(N)[:, None] + np.array(cols)[None, :]) % N
+    counts = np.array([40, 40, 30, 30, 30, 30])
+    prob = Problem(nbr, [30, 30, 8], counts, [4.0, 1.0, 0.5], np.zeros((3, K, K)))
+    s = prob.session(4, 16, 2, seed=1)
+    with pytest.raises(ValueError, match="64"):
+        s.guide(candidates=8, tau=0.1, accepts=1)
+    s.close()
