@@ -84,6 +84,20 @@ def parse():
     return ap.parse_args()
 
 
+def _host_cpu() -> str:
+    """CPU model and logical CPU count of this host (for the baseline's record)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
+
+
 def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0=None, fit0=None):
     """Reference engine (or the oracle when the reference binary is absent)
     running local search I on lattices of the same workload, one process per
@@ -98,9 +112,9 @@ def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0=None,
     kind = "reference" if RefHarness.available() else "port"
     start = "random"
     if lat0 is None:
-        # per-lattice cost of local search I measured on the reference engine
-        # in this repository's container: ~0.6 s for BCC 16^3, T = 256
-        per_lat = 0.6
+        # per-lattice cost of local search I on the reference engine, measured
+        # on an MI355X host: ~0.28 s of one core for BCC 16^3, T = 256
+        per_lat = 0.28
         n_lat = max(procs, int(round(target_secs / per_lat)))
         n_lat = (n_lat + procs - 1) // procs * procs
         lat0 = np.stack([o.permute(run["seed"], 0, 1, l) for l in range(n_lat)])
@@ -128,6 +142,7 @@ def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0=None,
         "value": st["evals"] / wall,
         "unit": "swap evals/s",
         "cores": procs,
+        "host": _host_cpu(),
         "kind": kind,
         "sample": f"local search I (T={T}, depth={D}) on {n_lat} {start} "
                   f"{spec.latt_typ} {spec.cell_dim}^3 lattices, "
