@@ -907,6 +907,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     if (s_me == s_long) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     // ---- X = sum_s w_s x_s, each wave's minimum and its candidate list
+    const double beta_c = T.sh->beta;  // loaded before the stores below
     double lmin = __builtin_inf();
     double X1 = 0.0;  // this lane's X when the round fits one pass
     bool has1 = false;
@@ -926,7 +927,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     {
       // this wave's proposals within 2 beta of the wave's minimum: a superset
       // of the final candidates, since the workgroup's minimum is no larger
-      const double wthr = lmin + 2.0 * T.sh->beta;
+      const double wthr = lmin + 2.0 * beta_c;
       if (task <= nthr) {  // one pass: lane tid holds proposal tid's X
         const bool f = has1 && !(X1 > wthr);
         const unsigned long long bal = __ballot(f);
@@ -962,6 +963,12 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // the exact stage is this workgroup's critical path: issue priority over
       // the waves of other workgroups on the CU until it ends
       __builtin_amdgcn_s_setprio(2);
+      // the round's state, loaded together with the wave minima (one LDS
+      // round trip; the fences below would keep later loads from moving up)
+      const double beta = T.sh->beta;
+      const int nsup = T.sh->ncl;
+      double prev_fit = T.sh->pf, fest = T.sh->fest;
+      int pfm = T.sh->pfm;
       double xmin = __builtin_inf();
 #pragma unroll
       for (int w2 = 0; w2 < kMaxWaves / 2; ++w2) {
@@ -969,11 +976,10 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         const double2 m2 = reinterpret_cast<const double2*>(T.sh->wmin)[w2];
         xmin = fmin(xmin, fmin(m2.x, m2.y));
       }
-      double thr = xmin + 2.0 * T.sh->beta;
+      double thr = xmin + 2.0 * beta;
       if (!(thr < __builtin_inf())) thr = __builtin_inf();
       // filter the waves' lists against the workgroup's window, compacting
       // in place (every entry is read before any of its chunk is written)
-      const int nsup = T.sh->ncl;
       int nc = 0, jfirst = -1;  // jfirst = cl[0] after the compaction
       for (int k0 = 0; k0 < nsup; k0 += 64) {
         const int j = (k0 + lane < nsup) ? T.cl[k0 + lane] : -1;
@@ -1002,9 +1008,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // One candidate with X < -2 beta is certainly accepted; X > 2 beta for
       // the smallest X is certainly a reject.  Near the threshold, or
       // otherwise, the exact path decides.
-      double prev_fit = T.sh->pf, fest = T.sh->fest;
-      int pfm = T.sh->pfm;
-      const double mg = 2.0 * T.sh->beta;
+      const double mg = 2.0 * beta;
       const bool near = A.thr > 0.0 && !(fest + xmin > A.thr * A.thr * 1.01);
       int kind = -1;
       if (pfm != 0 && !near) {
