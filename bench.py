@@ -73,7 +73,8 @@ def parse():
                     help="eager iterations timed per launch for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="approximate CPU work of the baseline sample (0 disables it)")
-    ap.add_argument("--cpu-procs", type=int, default=8)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
+                    help="CPU baseline processes (default: every host CPU, at most 16)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (default tile)")
     ap.add_argument("--tile", type=int, default=0,
@@ -230,8 +231,9 @@ def time_to_fitness(a):
         # evals plain mode needed; its rate is timed on lattices of the plain
         # run's final population (bounded: they are converged)
         procs = a.cpu_procs
+        n0 = min(L, 10 * procs)  # ~10-30 s of CPU work on converged lattices
         cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, procs,
-                           lat0=plain_final[0][:4 * procs], fit0=plain_final[1][:4 * procs])
+                           lat0=plain_final[0][:n0], fit0=plain_final[1][:n0])
         pm = modes["plain"]
         evals_to_thr = pm["evals"] * hp / pm["wall_s"]  # plain's rate is steady
         cpu["time_to_thr_projected_s"] = evals_to_thr / cpu["value"]
