@@ -90,7 +90,10 @@ int hea_problem_destroy(hea_problem p);
  * accelerate::gpu::run_local_parallel_hcs_cuda (pyhea/cpp/accelerate.h:23-32,
  * 51-60; pyhea/cpp/accelerate.cpp:713-763).  Arguments keep the reference's
  * meaning: solutions = nnet, total_iter = step, parallel_task = task
- * (1..2048), converge_depth = depth, thr only sets the time-to-threshold
+ * (1..2048; a round's per-proposal records must also fit the workgroup's
+ * 160 KiB of LDS, about 80 bytes per proposal for K <= 6 and 104 for K 7-8,
+ * so about 1900 and 1450 at most; larger rounds return HEA_EUNSUPPORTED),
+ * converge_depth = depth, thr only sets the time-to-threshold
  * statistic (the reference ignores it, accelerate.cpp:612, 718).  The final
  * population is sorted by ascending fitness.  out_lattices [solutions][N],
  * out_fitness [solutions], stats may be NULL. */

@@ -108,3 +108,32 @@ def test_local_mc_to_convergence_bit_exact(gpu, case):
     for k in ("evals", "rounds", "accepts", "draws"):
         assert sg[k] == so[k], k
     assert so["rounds"] > 8 * 4  # every lattice ran past its first rejections
+
+
+@pytest.mark.parametrize("case,task", [
+    # near the largest round whose per-proposal records fit 160 KiB of LDS
+    # (about 80 bytes per proposal), seven passes of 256
+    (("BCC", 4, [0.5, 0.3, 0.2], "true", 1), 1792),
+    (("FCC", 3, [0.4, 0.3, 0.3], "true", 3), 1200),  # five passes per round, three shells
+])
+def test_local_mc_largest_rounds_bit_exact(gpu, case, task):
+    spec = _spec(case, seed=17)
+    o, p = Oracle.from_spec(spec), Problem.from_spec(spec)
+    lat0 = np.stack([o.permute(41, 0, 0, l) for l in range(3)])
+    fit0 = o.fitness(lat0)
+    lo, fo, so = o.local_mc(lat0, fit0, task, 2, seed=41, iter=0, phase=2)
+    lg, fg, sg = p.local_mc(lat0, fit0, task, 2, seed=41, iter=0, phase=2)
+    assert np.array_equal(_bits(fg), _bits(fo))
+    assert np.array_equal(lg, lo)
+    for k in ("evals", "rounds", "accepts", "draws"):
+        assert sg[k] == so[k], k
+
+
+def test_round_too_large_for_lds_is_an_error(gpu):
+    """Per-proposal walk records of 2048 proposals in three shells do not fit
+    in 160 KiB of LDS: the call fails cleanly, it does not launch."""
+    spec = _spec(("FCC", 3, [0.4, 0.3, 0.3], "true", 3))
+    p = Problem.from_spec(spec)
+    lat0, fit0 = p.generate(2, seed=3)
+    with pytest.raises(ValueError, match="LDS"):
+        p.local_mc(lat0, fit0, 2048, 2, seed=3, iter=0, phase=2)
