@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=49)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain-bcc16",
-                    choices=["plain-bcc16", "plain-fcc10", "ttf-fcc30", "ttf-fcc50", "batched-64"])
+                    choices=["plain-bcc16", "plain-fcc10", "ttf-fcc30", "ttf-fcc50", "batched-64",
+                             "relax-fcc30", "relax-fcc50"])
     ap.add_argument("--budget-s", type=float, default=60.0,
                     help="time-to-fitness workloads: wall-clock budget per mode")
     ap.add_argument("--thr", type=float, default=1e-3)
@@ -294,6 +295,93 @@ def batched(a):
     print(json.dumps(out))
 
 
+def relax_bytes(n: int, sum_z: int, k: int, elem: int):
+    """Algorithmic bytes of one relaxation pass: the forward reads every
+    site's neighbour indices (int32) and occupancies; the backward reads the
+    same and writes the gradient."""
+    fwd = n * sum_z * 4 + n * k * elem
+    return fwd, fwd + n * k * elem
+
+
+def relaxation(a):
+    """Relaxation pass on one problem (FCC 30^3: 108000 sites, 5 species,
+    shells 12/6/24; or FCC 50^3: 500000 sites, 6 species): forward (softmax,
+    neighbour gather, deterministic FP64 pair-count reduction) and backward
+    (neighbour-gather adjoint) of the soft pair counts on float32 logits, the
+    kernels behind SoftPairCounts.  One step = one forward + one backward at
+    fixed synthetic logits and a fixed dn (the closed-form gradient of the
+    relaxed fitness at the first forward's counts).  Each call is timed with
+    HIP events on the stream it is launched on."""
+    import torch
+
+    from pyhea_amd import _native as Nat
+    from pyhea_amd.hcs import Problem
+    from pyhea_amd.problem import workload
+    from pyhea_amd.relax import RelaxObjective
+
+    wid = {"relax-fcc30": 3, "relax-fcc50": 4}[a.workload]
+    spec, _ = workload(wid)
+    prob = Problem.from_spec(spec)
+    N, K, S = spec.n_sites, spec.K, spec.S
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device="cpu").manual_seed(wid)
+    th = torch.randn((N, K), generator=g, dtype=torch.float32).to(dev)
+    p = torch.empty_like(th)
+    q = torch.empty((S, N, K), dtype=torch.float32, device=dev)
+    n = torch.empty((S, K, K), dtype=torch.float64, device=dev)
+    grad = torch.empty_like(th)
+    st = torch.cuda.current_stream()
+    h = prob.handle
+    lib = Nat.lib()
+
+    def fwd():
+        Nat.check(lib.hea_relax_forward(h, 0, th.data_ptr(), p.data_ptr(), q.data_ptr(),
+                                        n.data_ptr(), st.cuda_stream))
+
+    def bwd(dn):
+        Nat.check(lib.hea_relax_backward(h, 0, p.data_ptr(), q.data_ptr(), dn.data_ptr(),
+                                         grad.data_ptr(), st.cuda_stream))
+
+    fwd()
+    dn = RelaxObjective(prob, dev).count_gradient(n).contiguous()
+    for _ in range(max(a.warmup, 1)):
+        fwd()
+        bwd(dn)
+    steps = max(a.steps, 1)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e in ev:
+        e[0].record(st)
+        fwd()
+        e[1].record(st)
+        bwd(dn)
+        e[2].record(st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    f_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    b_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    bf, bb = relax_bytes(N, spec.sum_z, K, 4)
+    ach = (bf + bb) / ((f_ms + b_ms) * 1e-3) / 1e9
+    out = {
+        "metric": "relaxation passes/s", "value": steps / dt, "unit": "passes/s",
+        "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "lattice": f"{spec.latt_typ} {spec.cell_dim}^3",
+                   "n_sites": N, "species": K, "shells": spec.shell_sizes[:S]},
+        "forward_ms": round(f_ms, 4), "backward_ms": round(b_ms, 4),
+        "forward_gbs": round(bf / (f_ms * 1e-3) / 1e9, 1),
+        "backward_gbs": round(bb / (b_ms * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "relaxation forward + backward",
+                     "bytes_forward": bf, "bytes_backward": bb},
+    }
+    prob.close()
+    print(json.dumps(out))
+
+
 def main():
     if len(sys.argv) == 8 and sys.argv[1] == "--cpu-worker":
         cpu_worker(sys.argv[2], sys.argv[3], *map(int, sys.argv[4:8]))
@@ -321,6 +409,9 @@ def main():
         return
     if a.workload == "batched-64":
         batched(a)
+        return
+    if a.workload.startswith("relax-"):
+        relaxation(a)
         return
     from pyhea_amd import _native
     from pyhea_amd.hcs import Problem

@@ -205,7 +205,10 @@ int hea_batch_destroy(hea_batch b);
  * pair counts when p is one-hot.  All pointers are device memory; dtype 0 =
  * float32, 1 = float64 for logits, p [N][K], q [S][N][K] (neighbour sums
  * kept for the backward) and grad [N][K]; n and dn are float64 [S][K][K].
- * Asynchronous on `stream`.  Calls on one problem must not overlap. */
+ * Asynchronous on `stream`.  Calls on one problem must not overlap.  The
+ * first hea_relax_forward or hea_site_gain on a problem allocates its work
+ * buffers (about N * (4 * row ints + 64) bytes) and waits for `stream` once,
+ * so it must not be captured into a graph. */
 int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
                       void* d_n, void* stream);
 /* Neighbour-gather adjoint: d loss / d logits from dn = d loss / d n. */

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <utility>
 #include <functional>
 #include <string>
 #include <vector>
@@ -285,7 +286,10 @@ struct hea_problem_s {
   DevBuf<int32_t> d_counts, d_perm;
   DevBuf<int64_t> d_stats;
   DevBuf<uint32_t> d_iter;
-  DevBuf<double> d_relax_part;  // relaxation chunk partials
+  // relaxation: per-block pair-count partials, occupancies padded to G
+  // columns, and the slot-major copy of the neighbour table
+  DevBuf<double> d_relax_part, d_relax_pp;
+  DevBuf<int32_t> d_relax_nbrT;
 };
 
 struct hea_session_s {
@@ -1441,16 +1445,34 @@ int hea_batch_destroy(hea_batch b) {
 }  // extern "C"
 
 // ------------------------------------------------------------- relaxation --
-hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
-                                  int dtype, const void* th, void* p, void* Q, double* part,
-                                  double* n, hipStream_t st);
+hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
+                                  int dtype, const void* th, void* p, void* pp, void* Q,
+                                  double* part, double* n, hipStream_t st);
 hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
                                    const void* p, const void* Q, const double* dn, void* grad,
                                    hipStream_t st);
-hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
+hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                               const uint8_t* lattices, int L, const double* dn, float* u,
                               hipStream_t st);
-int hea_relax_chunks(int n_sites);
+size_t hea_relax_part_size(int n_sites, int K, int S);
+hipError_t hea_relax_transpose_nbr(const int32_t* nbr, int N, int stride, int32_t* nbrT,
+                                   hipStream_t st);
+
+// First relaxation call on a problem: work buffers and the slot-major copy
+// of the neighbour table (waits for `st` once).
+static int relax_prepare(hea_problem p, hipStream_t st) {
+  if (p->d_relax_nbrT.p) return HEA_OK;
+  const HeaConst& c = p->hc;
+  HEA_HIP(p->d_relax_part.alloc(hea_relax_part_size(c.N, c.K, c.S)));
+  HEA_HIP(p->d_relax_pp.alloc((size_t)c.N * 8));
+  DevBuf<int32_t> t;
+  HEA_HIP(t.alloc((size_t)c.N * c.stride));
+  HEA_HIP(hea_relax_transpose_nbr(p->d_nbr.p, c.N, c.stride, t.p, st));
+  HEA_HIP(hipStreamSynchronize(st));
+  std::swap(p->d_relax_nbrT.p, t.p);
+  std::swap(p->d_relax_nbrT.n, t.n);
+  return HEA_OK;
+}
 
 extern "C" {
 
@@ -1460,11 +1482,11 @@ int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* 
     return fail(HEA_EINVAL, "bad arguments");
   HEA_HIP(hipSetDevice(p->device));
   const HeaConst& c = p->hc;
-  const size_t need = (size_t)hea_relax_chunks(c.N) * c.S * c.K * c.K;
-  if (p->d_relax_part.n < need) HEA_HIP(p->d_relax_part.alloc(need));
-  HEA_HIP(hea_relax_forward_impl(p->d_const.p, c, p->d_nbr.p, dtype, d_logits, d_p, d_q,
-                                 p->d_relax_part.p, static_cast<double*>(d_n),
-                                 static_cast<hipStream_t>(stream)));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (int rc = relax_prepare(p, st)) return rc;
+  HEA_HIP(hea_relax_forward_impl(p->d_const.p, c, p->d_relax_nbrT.p, dtype, d_logits, d_p,
+                                 p->d_relax_pp.p, d_q, p->d_relax_part.p,
+                                 static_cast<double*>(d_n), st));
   return HEA_OK;
 }
 
@@ -1483,10 +1505,11 @@ int hea_site_gain(hea_problem p, int32_t L, const void* d_lattices, const void* 
                   void* stream) {
   if (!p || L < 1 || !d_lattices || !d_dn || !d_u) return fail(HEA_EINVAL, "bad arguments");
   HEA_HIP(hipSetDevice(p->device));
-  HEA_HIP(hea_site_gain_impl(p->d_const.p, p->hc, p->d_nbr.p,
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (int rc = relax_prepare(p, st)) return rc;
+  HEA_HIP(hea_site_gain_impl(p->d_const.p, p->hc, p->d_relax_nbrT.p,
                              static_cast<const uint8_t*>(d_lattices), L,
-                             static_cast<const double*>(d_dn), static_cast<float*>(d_u),
-                             static_cast<hipStream_t>(stream)));
+                             static_cast<const double*>(d_dn), static_cast<float*>(d_u), st));
   return HEA_OK;
 }
 

@@ -10,8 +10,14 @@
 // Warren-Cowley fitness smoothly.
 //
 // Forward: softmax, neighbour gather Q_s[i][b] = sum_{j in N_s(i)} p[j][b],
-// then n_s = P^T Q_s, reduced deterministically (fixed-size site chunks, then
-// chunk partials in order) in FP64.
+// then n_s = P^T Q_s, reduced deterministically in FP64 (a fixed-order sum
+// per block of sites, then the block partials in a fixed-shape tree).
+//
+// Layout: G lanes work on one site, one lane per species (G = 4 for K <= 4,
+// else 8), so a neighbour's occupancy row is one G-lane access of G * sizeof(T)
+// bytes (the softmax also writes p padded to G columns for this) and the
+// neighbour indices are read from a slot-major copy of the table
+// (nbrT[column][site]), which makes them one contiguous run per wave.
 // Backward (neighbour-gather adjoint): the neighbour relation is symmetric,
 // so dn[s][a][b]/dp[k][c] = [a=c] Q_s[k][b] + [b=c] Q_s[k][a] and
 //     g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q_s[k][b],
@@ -27,103 +33,164 @@
 
 namespace {
 
-constexpr int kChunk = 1024;  // sites per reduction chunk
+constexpr int kRelaxThreads = 1024;  // lanes per workgroup (SB = 1024 / G sites)
 
-template <typename T>
-__global__ void k_softmax(const T* __restrict__ th, T* __restrict__ p, int N, int K) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const T* r = th + (size_t)i * K;
-  double m = -1e300;
-  for (int a = 0; a < K; ++a) m = fmax(m, (double)r[a]);
-  double e[HEA_MAX_K], z = 0.0;
-  for (int a = 0; a < K; ++a) {
-    e[a] = exp((double)r[a] - m);
-    z += e[a];
-  }
-  for (int a = 0; a < K; ++a) p[(size_t)i * K + a] = (T)(e[a] / z);
+// Workgroups are dispatched round-robin over the eight XCDs; this bijection
+// of [0, nb) gives each XCD one contiguous range of logical blocks, so the
+// neighbour rows its sites read stay in its own L2.
+__device__ __forceinline__ int xcd_block(int w, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = w & 7, i = w >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-// Q[s][i][b] = sum over the shell-s neighbours j of i of p[j][b].
-template <typename T>
-__global__ void k_gather(const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
-                         const T* __restrict__ p, T* __restrict__ Q) {
+template <int G>
+__device__ __forceinline__ double group_max(double v) {
+  for (int o = 1; o < G; o <<= 1) v = fmax(v, __shfl_xor(v, o, G));
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+// nbrT[c][i] = nbr[i][c]: the slot-major copy of the neighbour table.
+__global__ void k_transpose_nbr(const int32_t* __restrict__ nbr, int N, int stride,
+                                int32_t* __restrict__ nbrT) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * stride) return;
+  const int c = (int)(t / N), i = (int)(t % N);
+  nbrT[t] = nbr[(size_t)i * stride + c];
+}
+
+// p[i][b] = softmax(theta[i])[b]; pp[i][G] is the same row padded with zeros.
+template <typename T, int G>
+__global__ __launch_bounds__(kRelaxThreads) void k_softmax(const T* __restrict__ th,
+                                                           T* __restrict__ p, T* __restrict__ pp,
+                                                           int N, int K) {
+  const int t = blockIdx.x * kRelaxThreads + threadIdx.x;
+  const int i = t / G, b = t % G;
+  if (i >= N) return;  // whole G-lane groups leave together
+  const bool on = b < K;
+  const double x = on ? (double)th[(size_t)i * K + b] : -HUGE_VAL;
+  const double m = group_max<G>(x);
+  const double e = on ? exp(x - m) : 0.0;
+  const double z = group_sum<G>(e);
+  const T v = on ? (T)(e / z) : (T)0;
+  if (on) p[(size_t)i * K + b] = v;
+  pp[(size_t)i * G + b] = v;
+}
+
+// Q[s][i][b] for one block of SB sites (lane b of site i sums species b over
+// i's shell-s neighbours, in list order), then the block's share of
+// n[s][a][b] = sum_i p[i][a] Q[s][i][b] from LDS: four interleaved site
+// subsets per entry, added in a fixed order.  part is [S K K][nb].
+template <typename T, int G>
+__global__ __launch_bounds__(kRelaxThreads) void k_gather_pairs(
+    const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbrT, const T* __restrict__ pp,
+    T* __restrict__ Q, double* __restrict__ part) {
+  constexpr int SB = kRelaxThreads / G;
+  __shared__ T sp[SB * G];
+  __shared__ T sq[HEA_MAX_S * SB * G];
+  __shared__ double red[4 * HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
   const HeaConst& P = *pc;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.N) return;
-  const int K = P.K;
-  const int32_t* row = nbr + (size_t)i * P.stride;
-  for (int s = 0; s < P.S; ++s) {
-    double acc[HEA_MAX_K];
-    for (int b = 0; b < K; ++b) acc[b] = 0.0;
-    for (int k = 0; k < P.Z[s]; ++k) {
-      const T* pj = p + (size_t)row[P.off[s] + k] * K;
-      for (int b = 0; b < K; ++b) acc[b] += (double)pj[b];
+  const int N = P.N, K = P.K, S = P.S;
+  const int nb = gridDim.x, blk = xcd_block(blockIdx.x, nb);
+  const int tl = threadIdx.x, il = tl / G, b = tl % G;
+  const int i = blk * SB + il;
+  if (i < N) {
+    sp[tl] = pp[(size_t)i * G + b];
+    for (int s = 0; s < S; ++s) {
+      const int32_t* col = nbrT + (size_t)P.off[s] * N + i;
+      const int Z = P.Z[s];
+      double acc = 0.0;
+      int k = 0;
+      for (; k + 4 <= Z; k += 4) {
+        const int j0 = col[(size_t)k * N], j1 = col[(size_t)(k + 1) * N];
+        const int j2 = col[(size_t)(k + 2) * N], j3 = col[(size_t)(k + 3) * N];
+        const T v0 = pp[(size_t)j0 * G + b], v1 = pp[(size_t)j1 * G + b];
+        const T v2 = pp[(size_t)j2 * G + b], v3 = pp[(size_t)j3 * G + b];
+        acc += (double)v0;
+        acc += (double)v1;
+        acc += (double)v2;
+        acc += (double)v3;
+      }
+      for (; k < Z; ++k) acc += (double)pp[(size_t)col[(size_t)k * N] * G + b];
+      const T q = (T)acc;
+      sq[s * SB * G + tl] = q;
+      if (b < K) Q[((size_t)s * N + i) * K + b] = q;
     }
-    T* q = Q + ((size_t)s * P.N + i) * K;
-    for (int b = 0; b < K; ++b) q[b] = (T)acc[b];
+  } else {
+    sp[tl] = (T)0;
+    for (int s = 0; s < S; ++s) sq[s * SB * G + tl] = (T)0;
   }
+  __syncthreads();
+  const int E = S * K * K;
+  if (tl < 4 * E) {
+    const int e = tl % E, r = tl / E;
+    const int s = e / (K * K), a = (e / K) % K, c = e % K;
+    const T* q = sq + s * SB * G;
+    double acc = 0.0;
+    for (int x = r; x < SB; x += 4) acc += (double)sp[x * G + a] * (double)q[x * G + c];
+    red[r * E + e] = acc;
+  }
+  __syncthreads();
+  if (tl < E) part[(size_t)tl * nb + blk] = ((red[tl] + red[E + tl]) + red[2 * E + tl]) + red[3 * E + tl];
 }
 
-// Partial n over one chunk of sites: part[chunk][s][a][b], one thread per
-// (s, a, b), summing the chunk's sites in order.
-template <typename T>
-__global__ void k_pair_partial(const HeaConst* __restrict__ pc, const T* __restrict__ p,
-                               const T* __restrict__ Q, double* __restrict__ part) {
-  const HeaConst& P = *pc;
-  const int K = P.K, S = P.S;
-  const int e = threadIdx.x;
-  if (e >= S * K * K) return;
-  const int s = e / (K * K), a = (e / K) % K, b = e % K;
-  const int i0 = blockIdx.x * kChunk, i1 = min(P.N, i0 + kChunk);
+// n[e] = sum over the nb block partials of entry e: strided lane sums, then
+// a fixed-shape tree (one workgroup per entry).
+__global__ __launch_bounds__(256) void k_pair_reduce(const double* __restrict__ part, int nb,
+                                                     double* __restrict__ n) {
+  __shared__ double r[256];
+  const double* row = part + (size_t)blockIdx.x * nb;
   double acc = 0.0;
-  for (int i = i0; i < i1; ++i)
-    acc += (double)p[(size_t)i * K + a] * (double)Q[((size_t)s * P.N + i) * K + b];
-  part[(size_t)blockIdx.x * S * K * K + e] = acc;
-}
-
-__global__ void k_pair_reduce(const double* __restrict__ part, int n_chunks, int n_ent,
-                              double* __restrict__ n) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_ent) return;
-  double acc = 0.0;
-  for (int c = 0; c < n_chunks; ++c) acc += part[(size_t)c * n_ent + e];
-  n[e] = acc;
+  for (int x = threadIdx.x; x < nb; x += 256) acc += row[x];
+  r[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) n[blockIdx.x] = r[0];
 }
 
 // grad[k][c] = p[k][c] (g[k][c] - sum_d p[k][d] g[k][d]),
-// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b].
-template <typename T>
-__global__ void k_adjoint(const HeaConst* __restrict__ pc, const T* __restrict__ p,
-                          const T* __restrict__ Q, const double* __restrict__ dn,
-                          T* __restrict__ grad) {
+// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; lane c of
+// site k's G-lane group computes g[k][c].
+template <typename T, int G>
+__global__ __launch_bounds__(kRelaxThreads) void k_adjoint(const HeaConst* __restrict__ pc,
+                                                           const T* __restrict__ p,
+                                                           const T* __restrict__ Q,
+                                                           const double* __restrict__ dn,
+                                                           T* __restrict__ grad) {
   const HeaConst& P = *pc;
-  const int K = P.K, S = P.S;
+  const int K = P.K, S = P.S, N = P.N;
   __shared__ double H[HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
   for (int e = threadIdx.x; e < S * K * K; e += blockDim.x) {
     const int s = e / (K * K), c = (e / K) % K, b = e % K;
     H[e] = dn[(s * K + c) * K + b] + dn[(s * K + b) * K + c];
   }
   __syncthreads();
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.N) return;
-  double g[HEA_MAX_K];
-  for (int c = 0; c < K; ++c) g[c] = 0.0;
+  const int t = blockIdx.x * kRelaxThreads + threadIdx.x;
+  const int k = t / G, c = t % G;
+  if (k >= N) return;  // whole G-lane groups leave together
+  const bool on = c < K;
+  const int cc = on ? c : 0;
+  double g = 0.0;
   for (int s = 0; s < S; ++s) {
-    const T* q = Q + ((size_t)s * P.N + k) * K;
-    for (int b = 0; b < K; ++b) {
-      const double qb = (double)q[b];
-      for (int c = 0; c < K; ++c) g[c] += H[(s * K + c) * K + b] * qb;
-    }
+    const T* q = Q + ((size_t)s * N + k) * K;
+    const double* h = H + (s * K + cc) * K;
+    for (int b = 0; b < K; ++b) g += h[b] * (double)q[b];
   }
-  double dot = 0.0;
-  for (int c = 0; c < K; ++c) dot += (double)p[(size_t)k * K + c] * g[c];
-  for (int c = 0; c < K; ++c)
-    grad[(size_t)k * K + c] = (T)((double)p[(size_t)k * K + c] * (g[c] - dot));
+  const double pk = on ? (double)p[(size_t)k * K + c] : 0.0;
+  const double dot = group_sum<G>(pk * g);
+  if (on) grad[(size_t)k * K + c] = (T)(pk * (g - dot));
 }
 
 // u[l][k][b] = g[k][b] - g[k][a_k] at the one-hot lattice l (see file header).
-__global__ void k_site_gain(const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
+__global__ void k_site_gain(const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbrT,
                             const uint8_t* __restrict__ lattices, const double* __restrict__ dn,
                             float* __restrict__ u) {
   const HeaConst& P = *pc;
@@ -139,12 +206,11 @@ __global__ void k_site_gain(const HeaConst* __restrict__ pc, const int32_t* __re
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= P.N) return;
   const uint8_t* sp = lattices + (size_t)l * P.N;
-  const int32_t* row = nbr + (size_t)k * P.stride;
   double g[HEA_MAX_K];
   for (int c = 0; c < K; ++c) g[c] = 0.0;
   for (int s = 0; s < S; ++s)
     for (int q = 0; q < P.Z[s]; ++q) {
-      const int b = sp[row[P.off[s] + q]];
+      const int b = sp[nbrT[(size_t)(P.off[s] + q) * P.N + k]];
       for (int c = 0; c < K; ++c) g[c] += H[(s * K + c) * K + b];
     }
   const int a = sp[k];
@@ -152,54 +218,74 @@ __global__ void k_site_gain(const HeaConst* __restrict__ pc, const int32_t* __re
   for (int b = 0; b < K; ++b) out[b] = (float)(g[b] - g[a]);
 }
 
-}  // namespace
+inline int relax_group(int K) { return K <= 4 ? 4 : 8; }
 
-int hea_relax_chunks(int n_sites) { return (n_sites + kChunk - 1) / kChunk; }
-
-template <typename T>
-static hipError_t relax_forward_t(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
-                                  const void* th, void* p, void* Q, double* part, double* n,
-                                  hipStream_t st) {
-  const int N = hc.N, K = hc.K, S = hc.S;
-  const int g = (N + 255) / 256;
-  hipLaunchKernelGGL(k_softmax<T>, dim3(g), dim3(256), 0, st, static_cast<const T*>(th),
-                     static_cast<T*>(p), N, K);
-  hipLaunchKernelGGL(k_gather<T>, dim3(g), dim3(256), 0, st, dc, nbr, static_cast<const T*>(p),
-                     static_cast<T*>(Q));
-  const int nch = hea_relax_chunks(N);
-  hipLaunchKernelGGL(k_pair_partial<T>, dim3(nch), dim3(256), 0, st, dc,
-                     static_cast<const T*>(p), static_cast<const T*>(Q), part);
-  const int ne = S * K * K;
-  hipLaunchKernelGGL(k_pair_reduce, dim3((ne + 63) / 64), dim3(64), 0, st, part, nch, ne, n);
+template <typename T, int G>
+hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
+                           const void* th, void* p, void* pp, void* Q, double* part, double* n,
+                           hipStream_t st) {
+  constexpr int SB = kRelaxThreads / G;
+  const int nb = (hc.N + SB - 1) / SB;
+  hipLaunchKernelGGL((k_softmax<T, G>), dim3(nb), dim3(kRelaxThreads), 0, st,
+                     static_cast<const T*>(th), static_cast<T*>(p), static_cast<T*>(pp), hc.N,
+                     hc.K);
+  hipLaunchKernelGGL((k_gather_pairs<T, G>), dim3(nb), dim3(kRelaxThreads), 0, st, dc, nbrT,
+                     static_cast<const T*>(pp), static_cast<T*>(Q), part);
+  hipLaunchKernelGGL(k_pair_reduce, dim3(hc.S * hc.K * hc.K), dim3(256), 0, st, part, nb, n);
   return hipGetLastError();
 }
 
-hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
-                                  int dtype, const void* th, void* p, void* Q, double* part,
-                                  double* n, hipStream_t st) {
-  return dtype == 1 ? relax_forward_t<double>(dc, hc, nbr, th, p, Q, part, n, st)
-                    : relax_forward_t<float>(dc, hc, nbr, th, p, Q, part, n, st);
+template <typename T, int G>
+hipError_t relax_backward_g(const HeaConst* dc, const HeaConst& hc, const void* p, const void* Q,
+                            const double* dn, void* grad, hipStream_t st) {
+  constexpr int SB = kRelaxThreads / G;
+  hipLaunchKernelGGL((k_adjoint<T, G>), dim3((hc.N + SB - 1) / SB), dim3(kRelaxThreads), 0, st,
+                     dc, static_cast<const T*>(p), static_cast<const T*>(Q), dn,
+                     static_cast<T*>(grad));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t hea_relax_part_size(int n_sites, int K, int S) {
+  const int SB = kRelaxThreads / relax_group(K);
+  return (size_t)((n_sites + SB - 1) / SB) * S * K * K;
+}
+
+hipError_t hea_relax_transpose_nbr(const int32_t* nbr, int N, int stride, int32_t* nbrT,
+                                   hipStream_t st) {
+  const size_t n = (size_t)N * stride;
+  hipLaunchKernelGGL(k_transpose_nbr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nbr, N,
+                     stride, nbrT);
+  return hipGetLastError();
+}
+
+hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
+                                  int dtype, const void* th, void* p, void* pp, void* Q,
+                                  double* part, double* n, hipStream_t st) {
+  const bool g4 = relax_group(hc.K) == 4;
+  if (dtype == 1)
+    return g4 ? relax_forward_g<double, 4>(dc, hc, nbrT, th, p, pp, Q, part, n, st)
+              : relax_forward_g<double, 8>(dc, hc, nbrT, th, p, pp, Q, part, n, st);
+  return g4 ? relax_forward_g<float, 4>(dc, hc, nbrT, th, p, pp, Q, part, n, st)
+            : relax_forward_g<float, 8>(dc, hc, nbrT, th, p, pp, Q, part, n, st);
 }
 
 hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
                                    const void* p, const void* Q, const double* dn, void* grad,
                                    hipStream_t st) {
-  const int g = (hc.N + 255) / 256;
+  const bool g4 = relax_group(hc.K) == 4;
   if (dtype == 1)
-    hipLaunchKernelGGL(k_adjoint<double>, dim3(g), dim3(256), 0, st, dc,
-                       static_cast<const double*>(p), static_cast<const double*>(Q), dn,
-                       static_cast<double*>(grad));
-  else
-    hipLaunchKernelGGL(k_adjoint<float>, dim3(g), dim3(256), 0, st, dc,
-                       static_cast<const float*>(p), static_cast<const float*>(Q), dn,
-                       static_cast<float*>(grad));
-  return hipGetLastError();
+    return g4 ? relax_backward_g<double, 4>(dc, hc, p, Q, dn, grad, st)
+              : relax_backward_g<double, 8>(dc, hc, p, Q, dn, grad, st);
+  return g4 ? relax_backward_g<float, 4>(dc, hc, p, Q, dn, grad, st)
+            : relax_backward_g<float, 8>(dc, hc, p, Q, dn, grad, st);
 }
 
-hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbr,
+hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                               const uint8_t* lattices, int L, const double* dn, float* u,
                               hipStream_t st) {
-  hipLaunchKernelGGL(k_site_gain, dim3((hc.N + 255) / 256, L), dim3(256), 0, st, dc, nbr,
+  hipLaunchKernelGGL(k_site_gain, dim3((hc.N + 255) / 256, L), dim3(256), 0, st, dc, nbrT,
                      lattices, dn, u);
   return hipGetLastError();
 }

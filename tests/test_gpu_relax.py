@@ -110,3 +110,49 @@ def test_site_gain_is_one_hot_gradient(gpu):
         want = g - g.gather(1, torch.as_tensor(lats[l], dtype=torch.long,
                                                device="cuda")[:, None])
         assert torch.allclose(u[l], want, rtol=1e-5, atol=1e-6 * want.abs().max().item())
+
+
+@pytest.mark.parametrize("typ,cell,K,S", [("BCC", 8, 4, 3), ("FCC", 5, 3, 2), ("FCC", 6, 7, 3)])
+def test_forward_backward_many_blocks(gpu, typ, cell, K, S):
+    """Lattices spanning several workgroups of sites (4 or 8 lanes per site,
+    last workgroup partly empty): counts and gradient in FP64 against the
+    PyTorch FP64 reference."""
+    spec = _spec(typ, cell, K, S, seed=cell)
+    prob = Problem.from_spec(spec)
+    obj = RelaxObjective(prob)
+    th0 = torch.randn(spec.n_sites, K, dtype=torch.float64, device="cuda")
+    ref_th = th0.clone().requires_grad_(True)
+    ref = torch_reference_counts(ref_th, spec.nbr, spec.shell_sizes)
+    th = th0.clone().requires_grad_(True)
+    n = SoftPairCounts.apply(th, prob)
+    assert torch.allclose(n, ref, rtol=1e-12, atol=1e-12)
+    obj.from_counts(n).backward()
+    obj.from_counts(ref).backward()
+    g, r = th.grad, ref_th.grad
+    assert (g - r).abs().max().item() <= 1e-10 * r.abs().max().item()
+
+
+@pytest.mark.parametrize("dtype,ntol,gtol", [(torch.float64, 1e-12, 1e-10),
+                                             (torch.float32, 2e-5, 2e-4)])
+def test_fcc30_full_size(gpu, dtype, ntol, gtol):
+    """FCC 30^3 (108000 sites, 5 species, shells 12/6/24), the time-to-fitness
+    problem: soft pair counts against the PyTorch FP64 reference, the
+    gradient against FP64 autograd, and the same bits on a second call."""
+    from pyhea_amd.problem import workload
+
+    spec, _ = workload(3)
+    prob = Problem.from_spec(spec)
+    obj = RelaxObjective(prob)
+    g0 = torch.Generator(device="cpu").manual_seed(30)
+    th0 = torch.randn((spec.n_sites, spec.K), generator=g0, dtype=torch.float64).cuda()
+    ref_th = th0.clone().requires_grad_(True)
+    ref = torch_reference_counts(ref_th, spec.nbr, spec.shell_sizes)
+    obj.from_counts(ref).backward()
+    th = th0.to(dtype).requires_grad_(True)
+    n = SoftPairCounts.apply(th, prob)
+    assert ((n - ref).abs() <= ntol * ref.abs().max()).all()
+    obj.from_counts(n).backward()
+    g, r = th.grad.double(), ref_th.grad
+    assert (g - r).abs().max().item() <= gtol * r.abs().max().item()
+    n2 = SoftPairCounts.apply(th.detach(), prob)
+    assert torch.equal(n, n2)
