@@ -38,15 +38,16 @@ def bytes_per_eval(sum_z: int) -> int:
     return 2 * sum_z * (4 + 1) + 2
 
 
-def profiled_traffic(workload: str):
-    """HBM bytes per local-search launch from the committed rocprofv3 byte
+def profiled_traffic(workload: str, kernel: str = "k_local_mc", key: str = "traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per step) from the committed rocprofv3 byte
     counters (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known
     size), or None when no profile covers the workload."""
-    path = os.path.join(ROOT, "profiles", f"k_local_mc_{workload.replace('-', '_')}.md")
+    name = workload.replace("-", "_").replace("relax_", "")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{name}.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return int(line.split(":")[1])
     except OSError:
         pass
@@ -374,7 +375,8 @@ def relaxation(a):
         "forward_gbs": round(bf / (f_ms * 1e-3) / 1e9, 1),
         "backward_gbs": round(bb / (b_ms * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(a.workload, "k_relax", "traffic_bytes_per_step"),
                      "kernel": "relaxation forward + backward",
                      "bytes_forward": bf, "bytes_backward": bb},
     }

@@ -13,7 +13,7 @@
 // then n_s = P^T Q_s, reduced deterministically in FP64 (a fixed-order sum
 // per block of sites, then the block partials in a fixed-shape tree).
 //
-// Layout: G lanes work on one site, one lane per species (G = 4 for K <= 4,
+// Forward layout: G lanes work on one site, one lane per species (G = 4 for K <= 4,
 // else 8), so a neighbour's occupancy row is one G-lane access of G * sizeof(T)
 // bytes (the softmax also writes p padded to G columns for this) and the
 // neighbour indices are read from a slot-major copy of the table
@@ -157,36 +157,52 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const double* __restrict__ 
 }
 
 // grad[k][c] = p[k][c] (g[k][c] - sum_d p[k][d] g[k][d]),
-// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; lane c of
-// site k's G-lane group computes g[k][c].
-template <typename T, int G>
-__global__ __launch_bounds__(kRelaxThreads) void k_adjoint(const HeaConst* __restrict__ pc,
-                                                           const T* __restrict__ p,
-                                                           const T* __restrict__ Q,
-                                                           const double* __restrict__ dn,
-                                                           T* __restrict__ grad) {
+// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; one lane per
+// site, whose p and Q rows are loaded before the workgroup builds the
+// symmetrised table H in LDS, so the loads overlap that step.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adjoint(const HeaConst* __restrict__ pc,
+                                                 const T* __restrict__ p,
+                                                 const T* __restrict__ Q,
+                                                 const double* __restrict__ dn,
+                                                 T* __restrict__ grad) {
   const HeaConst& P = *pc;
   const int K = P.K, S = P.S, N = P.N;
   __shared__ double H[HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const bool live = k < N;
+  T q[HEA_MAX_S][HEA_MAX_K], pr[HEA_MAX_K];
+#pragma unroll
+  for (int s = 0; s < HEA_MAX_S; ++s)
+#pragma unroll
+    for (int b = 0; b < HEA_MAX_K; ++b)
+      q[s][b] = (live && s < S && b < K) ? Q[((size_t)s * N + k) * K + b] : (T)0;
+#pragma unroll
+  for (int b = 0; b < HEA_MAX_K; ++b) pr[b] = (live && b < K) ? p[(size_t)k * K + b] : (T)0;
   for (int e = threadIdx.x; e < S * K * K; e += blockDim.x) {
     const int s = e / (K * K), c = (e / K) % K, b = e % K;
     H[e] = dn[(s * K + c) * K + b] + dn[(s * K + b) * K + c];
   }
   __syncthreads();
-  const int t = blockIdx.x * kRelaxThreads + threadIdx.x;
-  const int k = t / G, c = t % G;
-  if (k >= N) return;  // whole G-lane groups leave together
-  const bool on = c < K;
-  const int cc = on ? c : 0;
-  double g = 0.0;
-  for (int s = 0; s < S; ++s) {
-    const T* q = Q + ((size_t)s * N + k) * K;
-    const double* h = H + (s * K + cc) * K;
-    for (int b = 0; b < K; ++b) g += h[b] * (double)q[b];
+  if (!live) return;
+  double g[HEA_MAX_K];
+#pragma unroll
+  for (int c = 0; c < HEA_MAX_K; ++c) {
+    double a = 0.0;
+#pragma unroll
+    for (int s = 0; s < HEA_MAX_S; ++s)
+#pragma unroll
+      for (int b = 0; b < HEA_MAX_K; ++b)
+        if (c < K && s < S && b < K) a += H[(s * K + c) * K + b] * (double)q[s][b];
+    g[c] = a;
   }
-  const double pk = on ? (double)p[(size_t)k * K + c] : 0.0;
-  const double dot = group_sum<G>(pk * g);
-  if (on) grad[(size_t)k * K + c] = (T)(pk * (g - dot));
+  double dot = 0.0;
+#pragma unroll
+  for (int c = 0; c < HEA_MAX_K; ++c)
+    if (c < K) dot += (double)pr[c] * g[c];
+#pragma unroll
+  for (int c = 0; c < HEA_MAX_K; ++c)
+    if (c < K) grad[(size_t)k * K + c] = (T)((double)pr[c] * (g[c] - dot));
 }
 
 // u[l][k][b] = g[k][b] - g[k][a_k] at the one-hot lattice l (see file header).
@@ -235,12 +251,11 @@ hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t
   return hipGetLastError();
 }
 
-template <typename T, int G>
-hipError_t relax_backward_g(const HeaConst* dc, const HeaConst& hc, const void* p, const void* Q,
+template <typename T>
+hipError_t relax_backward_t(const HeaConst* dc, const HeaConst& hc, const void* p, const void* Q,
                             const double* dn, void* grad, hipStream_t st) {
-  constexpr int SB = kRelaxThreads / G;
-  hipLaunchKernelGGL((k_adjoint<T, G>), dim3((hc.N + SB - 1) / SB), dim3(kRelaxThreads), 0, st,
-                     dc, static_cast<const T*>(p), static_cast<const T*>(Q), dn,
+  hipLaunchKernelGGL((k_adjoint<T>), dim3((hc.N + 255) / 256), dim3(256), 0, st, dc,
+                     static_cast<const T*>(p), static_cast<const T*>(Q), dn,
                      static_cast<T*>(grad));
   return hipGetLastError();
 }
@@ -274,12 +289,8 @@ hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const 
 hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
                                    const void* p, const void* Q, const double* dn, void* grad,
                                    hipStream_t st) {
-  const bool g4 = relax_group(hc.K) == 4;
-  if (dtype == 1)
-    return g4 ? relax_backward_g<double, 4>(dc, hc, p, Q, dn, grad, st)
-              : relax_backward_g<double, 8>(dc, hc, p, Q, dn, grad, st);
-  return g4 ? relax_backward_g<float, 4>(dc, hc, p, Q, dn, grad, st)
-            : relax_backward_g<float, 8>(dc, hc, p, Q, dn, grad, st);
+  return dtype == 1 ? relax_backward_t<double>(dc, hc, p, Q, dn, grad, st)
+                    : relax_backward_t<float>(dc, hc, p, Q, dn, grad, st);
 }
 
 hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
