@@ -156,3 +156,53 @@ def test_fcc30_full_size(gpu, dtype, ntol, gtol):
     assert (g - r).abs().max().item() <= gtol * r.abs().max().item()
     n2 = SoftPairCounts.apply(th.detach(), prob)
     assert torch.equal(n, n2)
+
+
+@pytest.mark.parametrize("typ,cell,K", [("BCC", 8, 5), ("FCC", 5, 2)])
+def test_site_gain_many_blocks(gpu, typ, cell, K):
+    """Site gains on lattices spanning several workgroups of sites (two
+    lattices per call) against the FP64 autograd gradient at the one-hot
+    point."""
+    spec = _spec(typ, cell, K, 3, seed=11)
+    prob, o = Problem.from_spec(spec), Oracle.from_spec(spec)
+    obj = RelaxObjective(prob)
+    lats = np.stack([o.permute(12, 0, 0, l) for l in range(2)])
+    dn = torch.stack([obj.count_gradient(torch.as_tensor(o.pair_counts(l), dtype=torch.float64,
+                                                          device="cuda")) for l in lats])
+    u = site_gain(prob, torch.as_tensor(lats, device="cuda"), dn).double()
+    for l in range(2):
+        p = torch.nn.functional.one_hot(torch.as_tensor(lats[l], dtype=torch.long), K)
+        p = p.double().cuda().requires_grad_(True)
+        obj.from_counts(torch_reference_counts_p(p, spec.nbr, spec.shell_sizes)).backward()
+        g = p.grad
+        want = g - g.gather(1, torch.as_tensor(lats[l], dtype=torch.long,
+                                               device="cuda")[:, None])
+        assert torch.allclose(u[l], want, rtol=1e-5, atol=1e-6 * want.abs().max().item())
+
+
+def torch_reference_counts_p(p, nbr, shell_sizes):
+    out, off = [], 0
+    for z in shell_sizes:
+        idx = torch.as_tensor(nbr[:, off:off + z], dtype=torch.long, device=p.device)
+        out.append(p.T @ p[idx].sum(1))
+        off += z
+    return torch.stack(out)
+
+
+@pytest.mark.parametrize("typ,cell,K,S", [("FCC", 6, 8, 3), ("BCC", 7, 2, 3)])
+def test_forward_backward_extreme_species(gpu, typ, cell, K, S):
+    """The most species the kernels take (8) and the fewest (2), several
+    workgroups of sites, FP64 against the PyTorch reference."""
+    spec = _spec(typ, cell, K, S, seed=K)
+    prob = Problem.from_spec(spec)
+    obj = RelaxObjective(prob)
+    th0 = torch.randn(spec.n_sites, K, dtype=torch.float64, device="cuda")
+    ref_th = th0.clone().requires_grad_(True)
+    ref = torch_reference_counts(ref_th, spec.nbr, spec.shell_sizes)
+    th = th0.clone().requires_grad_(True)
+    n = SoftPairCounts.apply(th, prob)
+    assert torch.allclose(n, ref, rtol=1e-12, atol=1e-12)
+    obj.from_counts(n).backward()
+    obj.from_counts(ref).backward()
+    g, r = th.grad, ref_th.grad
+    assert (g - r).abs().max().item() <= 1e-10 * r.abs().max().item()
