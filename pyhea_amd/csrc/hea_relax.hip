@@ -13,11 +13,11 @@
 // then n_s = P^T Q_s, reduced deterministically in FP64 (a fixed-order sum
 // per block of sites, then the block partials in a fixed-shape tree).
 //
-// Forward layout: G lanes work on one site, one lane per species (G = 4 for K <= 4,
-// else 8), so a neighbour's occupancy row is one G-lane access of G * sizeof(T)
-// bytes (the softmax also writes p padded to G columns for this) and the
-// neighbour indices are read from a slot-major copy of the table
-// (nbrT[column][site]), which makes them one contiguous run per wave.
+// Forward layout: the softmax writes p padded to KP columns (KP = 4 for
+// K <= 4, else 8); the gather gives each site KP / 4 lanes of four species
+// each, so a neighbour's row is one 4-element vector load per lane (16 bytes
+// in float32), and reads the neighbour indices from a slot-major copy of the
+// table (nbrT[column][site]), one contiguous run per wave.
 // Backward (neighbour-gather adjoint): the neighbour relation is symmetric,
 // so dn[s][a][b]/dp[k][c] = [a=c] Q_s[k][b] + [b=c] Q_s[k][a] and
 //     g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q_s[k][b],
@@ -33,7 +33,9 @@
 
 namespace {
 
-constexpr int kRelaxThreads = 1024;  // lanes per workgroup (SB = 1024 / G sites)
+constexpr int kRelaxThreads = 1024;  // softmax lanes per workgroup (KP lanes per site)
+constexpr int kGatherThreads = 512;  // gather lanes per workgroup
+constexpr int kGatherVec = 4;        // species per gather lane
 
 // Workgroups are dispatched round-robin over the eight XCDs; this bijection
 // of [0, nb) gives each XCD one contiguous range of logical blocks, so the
@@ -64,7 +66,8 @@ __global__ void k_transpose_nbr(const int32_t* __restrict__ nbr, int N, int stri
   nbrT[t] = nbr[(size_t)i * stride + c];
 }
 
-// p[i][b] = softmax(theta[i])[b]; pp[i][G] is the same row padded with zeros.
+// p[i][b] = softmax(theta[i])[b], G lanes per site (one per column);
+// pp[i][G] is the same row padded with zeros.
 template <typename T, int G>
 __global__ __launch_bounds__(kRelaxThreads) void k_softmax(const T* __restrict__ th,
                                                            T* __restrict__ p, T* __restrict__ pp,
@@ -82,61 +85,91 @@ __global__ __launch_bounds__(kRelaxThreads) void k_softmax(const T* __restrict__
   pp[(size_t)i * G + b] = v;
 }
 
-// Q[s][i][b] for one block of SB sites (lane b of site i sums species b over
-// i's shell-s neighbours, in list order), then the block's share of
-// n[s][a][b] = sum_i p[i][a] Q[s][i][b] from LDS: four interleaved site
-// subsets per entry, added in a fixed order.  part is [S K K][nb].
-template <typename T, int G>
-__global__ __launch_bounds__(kRelaxThreads) void k_gather_pairs(
+template <typename T, int V>
+struct alignas(V * sizeof(T)) VecT {
+  T x[V];
+};
+
+// Q[s][i][c] for one block of SB sites: lane b of site i sums species
+// b*V .. b*V+V-1 over i's shell-s neighbours, in list order; then the
+// block's share of n[s][a][c] = sum_i p[i][a] Q[s][i][c] from LDS: four
+// interleaved site subsets per entry, added in a fixed order.  Rows of pp
+// have G*V columns; part is [S K K][nb].
+template <typename T, int G, int V>
+__global__ __launch_bounds__(kGatherThreads) void k_gather_pairs(
     const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbrT, const T* __restrict__ pp,
     T* __restrict__ Q, double* __restrict__ part) {
-  constexpr int SB = kRelaxThreads / G;
-  __shared__ T sp[SB * G];
-  __shared__ T sq[HEA_MAX_S * SB * G];
+  constexpr int SB = kGatherThreads / G, KP = G * V;
+  using Vec = VecT<T, V>;
+  __shared__ T sp[SB * KP];
+  __shared__ T sq[HEA_MAX_S * SB * KP];
   __shared__ double red[4 * HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
   const HeaConst& P = *pc;
   const int N = P.N, K = P.K, S = P.S;
   const int nb = gridDim.x, blk = xcd_block(blockIdx.x, nb);
   const int tl = threadIdx.x, il = tl / G, b = tl % G;
   const int i = blk * SB + il;
+  T* myp = sp + il * KP + b * V;
   if (i < N) {
-    sp[tl] = pp[(size_t)i * G + b];
+    const Vec own = *reinterpret_cast<const Vec*>(pp + (size_t)i * KP + b * V);
+#pragma unroll
+    for (int v = 0; v < V; ++v) myp[v] = own.x[v];
     for (int s = 0; s < S; ++s) {
       const int32_t* col = nbrT + (size_t)P.off[s] * N + i;
+      const T* row = pp + b * V;
       const int Z = P.Z[s];
-      double acc = 0.0;
+      double acc[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] = 0.0;
       int k = 0;
       for (; k + 4 <= Z; k += 4) {
         const int j0 = col[(size_t)k * N], j1 = col[(size_t)(k + 1) * N];
         const int j2 = col[(size_t)(k + 2) * N], j3 = col[(size_t)(k + 3) * N];
-        const T v0 = pp[(size_t)j0 * G + b], v1 = pp[(size_t)j1 * G + b];
-        const T v2 = pp[(size_t)j2 * G + b], v3 = pp[(size_t)j3 * G + b];
-        acc += (double)v0;
-        acc += (double)v1;
-        acc += (double)v2;
-        acc += (double)v3;
+        const Vec v0 = *reinterpret_cast<const Vec*>(row + (size_t)j0 * KP);
+        const Vec v1 = *reinterpret_cast<const Vec*>(row + (size_t)j1 * KP);
+        const Vec v2 = *reinterpret_cast<const Vec*>(row + (size_t)j2 * KP);
+        const Vec v3 = *reinterpret_cast<const Vec*>(row + (size_t)j3 * KP);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          acc[v] += (double)v0.x[v];
+          acc[v] += (double)v1.x[v];
+          acc[v] += (double)v2.x[v];
+          acc[v] += (double)v3.x[v];
+        }
       }
-      for (; k < Z; ++k) acc += (double)pp[(size_t)col[(size_t)k * N] * G + b];
-      const T q = (T)acc;
-      sq[s * SB * G + tl] = q;
-      if (b < K) Q[((size_t)s * N + i) * K + b] = q;
+      for (; k < Z; ++k) {
+        const Vec vk = *reinterpret_cast<const Vec*>(row + (size_t)col[(size_t)k * N] * KP);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += (double)vk.x[v];
+      }
+      T* myq = sq + s * SB * KP + il * KP + b * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const T q = (T)acc[v];
+        myq[v] = q;
+        if (b * V + v < K) Q[((size_t)s * N + i) * K + b * V + v] = q;
+      }
     }
   } else {
-    sp[tl] = (T)0;
-    for (int s = 0; s < S; ++s) sq[s * SB * G + tl] = (T)0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) myp[v] = (T)0;
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int v = 0; v < V; ++v) sq[s * SB * KP + il * KP + b * V + v] = (T)0;
   }
   __syncthreads();
   const int E = S * K * K;
-  if (tl < 4 * E) {
-    const int e = tl % E, r = tl / E;
+  for (int t = tl; t < 4 * E; t += kGatherThreads) {
+    const int e = t % E, r = t / E;
     const int s = e / (K * K), a = (e / K) % K, c = e % K;
-    const T* q = sq + s * SB * G;
+    const T* q = sq + s * SB * KP;
     double acc = 0.0;
-    for (int x = r; x < SB; x += 4) acc += (double)sp[x * G + a] * (double)q[x * G + c];
+    for (int x = r; x < SB; x += 4) acc += (double)sp[x * KP + a] * (double)q[x * KP + c];
     red[r * E + e] = acc;
   }
   __syncthreads();
-  if (tl < E) part[(size_t)tl * nb + blk] = ((red[tl] + red[E + tl]) + red[2 * E + tl]) + red[3 * E + tl];
+  for (int e = tl; e < E; e += kGatherThreads)
+    part[(size_t)e * nb + blk] = ((red[e] + red[E + e]) + red[2 * E + e]) + red[3 * E + e];
 }
 
 // n[e] = sum over the nb block partials of entry e: strided lane sums, then
@@ -234,19 +267,27 @@ __global__ void k_site_gain(const HeaConst* __restrict__ pc, const int32_t* __re
   for (int b = 0; b < K; ++b) out[b] = (float)(g[b] - g[a]);
 }
 
-inline int relax_group(int K) { return K <= 4 ? 4 : 8; }
+// Columns of the padded occupancy rows.
+inline int relax_cols(int K) { return K <= 4 ? 4 : 8; }
 
-template <typename T, int G>
+// Workgroups of the gather (sites per workgroup: kGatherThreads / (KP / 4)).
+inline int relax_gather_blocks(int N, int K) {
+  const int sb = kGatherThreads / (relax_cols(K) / kGatherVec);
+  return (N + sb - 1) / sb;
+}
+
+template <typename T, int KP>
 hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                            const void* th, void* p, void* pp, void* Q, double* part, double* n,
                            hipStream_t st) {
-  constexpr int SB = kRelaxThreads / G;
-  const int nb = (hc.N + SB - 1) / SB;
-  hipLaunchKernelGGL((k_softmax<T, G>), dim3(nb), dim3(kRelaxThreads), 0, st,
-                     static_cast<const T*>(th), static_cast<T*>(p), static_cast<T*>(pp), hc.N,
+  constexpr int SBS = kRelaxThreads / KP;
+  hipLaunchKernelGGL((k_softmax<T, KP>), dim3((hc.N + SBS - 1) / SBS), dim3(kRelaxThreads), 0,
+                     st, static_cast<const T*>(th), static_cast<T*>(p), static_cast<T*>(pp), hc.N,
                      hc.K);
-  hipLaunchKernelGGL((k_gather_pairs<T, G>), dim3(nb), dim3(kRelaxThreads), 0, st, dc, nbrT,
-                     static_cast<const T*>(pp), static_cast<T*>(Q), part);
+  const int nb = relax_gather_blocks(hc.N, hc.K);
+  hipLaunchKernelGGL((k_gather_pairs<T, KP / kGatherVec, kGatherVec>), dim3(nb),
+                     dim3(kGatherThreads), 0, st, dc, nbrT, static_cast<const T*>(pp),
+                     static_cast<T*>(Q), part);
   hipLaunchKernelGGL(k_pair_reduce, dim3(hc.S * hc.K * hc.K), dim3(256), 0, st, part, nb, n);
   return hipGetLastError();
 }
@@ -263,8 +304,7 @@ hipError_t relax_backward_t(const HeaConst* dc, const HeaConst& hc, const void* 
 }  // namespace
 
 size_t hea_relax_part_size(int n_sites, int K, int S) {
-  const int SB = kRelaxThreads / relax_group(K);
-  return (size_t)((n_sites + SB - 1) / SB) * S * K * K;
+  return (size_t)relax_gather_blocks(n_sites, K) * S * K * K;
 }
 
 hipError_t hea_relax_transpose_nbr(const int32_t* nbr, int N, int stride, int32_t* nbrT,
@@ -278,7 +318,7 @@ hipError_t hea_relax_transpose_nbr(const int32_t* nbr, int N, int stride, int32_
 hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                                   int dtype, const void* th, void* p, void* pp, void* Q,
                                   double* part, double* n, hipStream_t st) {
-  const bool g4 = relax_group(hc.K) == 4;
+  const bool g4 = relax_cols(hc.K) == 4;
   if (dtype == 1)
     return g4 ? relax_forward_g<double, 4>(dc, hc, nbrT, th, p, pp, Q, part, n, st)
               : relax_forward_g<double, 8>(dc, hc, nbrT, th, p, pp, Q, part, n, st);
