@@ -80,7 +80,10 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (default tile)")
     ap.add_argument("--tile", type=int, default=0,
-                    help="plain workloads: proposals per pass (64..256) instead of the autotuner's")
+                    help="plain and batched workloads: proposals per pass (64..256) instead of "
+                         "the autotuner's")
+    ap.add_argument("--grid", type=int, default=0,
+                    help="batched workload: persistent workgroups instead of the autotuner's")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a copy "
                          "kernel of known size for calibrating HBM byte counters)")
@@ -258,11 +261,13 @@ def batched(a):
     specs = batched_workload()
     probs = [Problem.from_spec(sp) for sp in specs]
     L, T, D, seed = 128, 256, 10, 2002
-    shape = {"grid": None, "tile": None}
-    if not a.no_tune:
+    shape = {"grid": a.grid or None, "tile": a.tile or None}
+    if not a.no_tune and not (a.grid and a.tile):
         from pyhea_amd.autotune import tune_batch
 
-        shape = tune_batch(probs, L, T, D, seed=seed, cache_path=None)
+        t = tune_batch(probs, L, T, D, seed=seed, cache_path=None,
+                       grids=[a.grid] if a.grid else None, tiles=[a.tile] if a.tile else None)
+        shape = {"grid": t["grid"], "tile": t["tile"]}
     b = Batch(probs, L, T, D, seed=seed)
     if shape["tile"]:
         b.tile(shape["tile"])
@@ -291,7 +296,8 @@ def batched(a):
                    "converge_depth": D, "seed": seed, "grid": shape["grid"], "tile": shape["tile"]},
         "roofline": {"bound": "hbm", "achieved": round(byts / dt / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(byts / dt / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_local_mc_batch (whole iterations)"},
+                     "traffic": profiled_traffic(a.workload, "k_local_mc_batch"),
+                     "kernel": "k_local_mc_batch (whole iterations)"},
     }
     print(json.dumps(out))
 
