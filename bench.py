@@ -13,6 +13,11 @@ Prints one JSON line: swap evals/s (value), the roofline of the dominant
 kernel (the local-search kernel, timed live with HIP events on its stream)
 and a CPU baseline: the reference engine (oracle/_ref/ref_hcs) timed on a
 bounded sample of the same workload on the host cores.
+
+Other workloads (--workload): plain-fcc10; ttf-fcc30 / ttf-fcc50 (time to
+fitness, plain vs guided vs guided-batched); batched-64 (64 uneven problems
+in one persistent launch per phase); relax-fcc30 / relax-fcc50 (forward and
+backward of the differentiable relaxation).
 """
 from __future__ import annotations
 
