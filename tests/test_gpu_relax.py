@@ -206,3 +206,24 @@ def test_forward_backward_extreme_species(gpu, typ, cell, K, S):
     obj.from_counts(ref).backward()
     g, r = th.grad, ref_th.grad
     assert (g - r).abs().max().item() <= 1e-10 * r.abs().max().item()
+
+
+def test_side_stream_gives_same_bits(gpu):
+    """Forward and backward launched on a side stream (the first relaxation
+    call of the problem among them) give the default stream's bits."""
+    spec = _spec("FCC", 5, 5, 3, seed=21)
+    th0 = torch.randn(spec.n_sites, 5, dtype=torch.float32, device="cuda")
+    out = []
+    for side in (True, False):
+        prob = Problem.from_spec(spec)
+        obj = RelaxObjective(prob)
+        th = th0.clone().requires_grad_(True)
+        s = torch.cuda.Stream() if side else torch.cuda.current_stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            n = SoftPairCounts.apply(th, prob)
+            obj.from_counts(n).backward()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        out.append((n.detach().clone(), th.grad.clone()))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
