@@ -82,3 +82,26 @@ def test_autotuner_picks_a_valid_shape(gpu, tmp_path):
     assert best["tile"] in (64, 128) and best["grid"] in (4, 8)
     again = tune_batch(probs, 4, 64, 2, cache_path=str(tmp_path / "t.json"))
     assert again == best
+
+
+@pytest.mark.parametrize("K", [5, 6, 8])
+def test_uniform_species_batch_equals_plain(gpu, K):
+    """Batches whose problems all have K equal to the batch kernel's table
+    width (5, 6, 8): each problem equals its plain run, which uses the
+    exact-K single-problem kernel."""
+    rng = np.random.default_rng(K)
+    specs = [make_problem(typ, cell, rng.dirichlet([2.0] * K) + 0.05, shells="true",
+                          target_seed=80 + i)
+             for i, (typ, cell) in enumerate([("FCC", 3), ("BCC", 4), ("FCC", 4)])]
+    probs = [Problem.from_spec(s) for s in specs]
+    L, I, T, D, seed = 4, 2, 64, 2, 910
+    b = Batch(probs, L, T, D, seed=seed)
+    b.init()
+    b.step(I)
+    for q, p in enumerate(probs):
+        lb, fb, sb = b.read(q)
+        lp, fp, sp = p.run_hcs(L, I, T, D, seed=seed + q)
+        assert np.array_equal(lb, lp), q
+        assert np.array_equal(_bits(fb), _bits(fp)), q
+        assert sb["evals"] == sp["evals"] and sb["accepts"] == sp["accepts"], q
+    b.close()

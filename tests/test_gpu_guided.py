@@ -79,3 +79,23 @@ def test_guide_rejects_records_wider_than_64_bits(gpu):
     with pytest.raises(ValueError, match="64"):
         s.guide(candidates=8, tau=0.1, accepts=1)
     s.close()
+
+
+@pytest.mark.parametrize("K", [5, 6])
+def test_guided_uniform_species_batch_matches_sessions(gpu, K):
+    """Guided batch whose problems all have K equal to the table width
+    against guided sessions of each problem (exact-K kernels)."""
+    specs = [_spec("FCC", 3, K, 31), _spec("BCC", 4, K, 32)]
+    probs = [Problem.from_spec(s) for s in specs]
+    b = Batch(probs, 4, 32, 2, seed=71)
+    b.guide(candidates=8, tau=0.1, accepts=3)
+    b.init()
+    b.step(2)
+    for q, p in enumerate(probs):
+        s = p.session(4, 32, 2, seed=71 + q)
+        s.guide(candidates=8, tau=0.1, accepts=3)
+        s.init()
+        s.step(2)
+        ls, fs, _ = s.read()
+        lb, fb, _ = b.read(q)
+        assert np.array_equal(lb, ls) and np.array_equal(fb, fs)
