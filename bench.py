@@ -375,6 +375,9 @@ def relaxation(a):
     b_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
     bf, bb = relax_bytes(N, spec.sum_z, K, 4)
     ach = (bf + bb) / ((f_ms + b_ms) * 1e-3) / 1e9
+    cpu = None
+    if a.cpu_seconds > 0:
+        cpu = relax_cpu_baseline(spec, th.cpu(), dn.cpu(), min(a.cpu_seconds, 10.0), a.cpu_procs)
     out = {
         "metric": "relaxation passes/s", "value": steps / dt, "unit": "passes/s",
         "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / steps,
@@ -390,9 +393,40 @@ def relaxation(a):
                      "traffic": profiled_traffic(a.workload, "k_relax", "traffic_bytes_per_step"),
                      "kernel": "relaxation forward + backward",
                      "bytes_forward": bf, "bytes_backward": bb},
+        "cpu_baseline": cpu,
     }
     prob.close()
     print(json.dumps(out))
+
+
+def relax_cpu_baseline(spec, th, dn, target_secs: float, threads: int):
+    """The same float32 forward + backward (softmax, neighbour sums,
+    n_s = P^T Q_s, gradient of <n, dn>) in PyTorch on the host CPU, repeated
+    for about target_secs (the relaxation has no counterpart in the
+    reference engine)."""
+    import torch
+
+    torch.set_num_threads(max(1, threads))
+    idx = []
+    off = 0
+    for z in spec.shell_sizes[:spec.S]:
+        idx.append(torch.as_tensor(np.ascontiguousarray(spec.nbr[:, off:off + z]), dtype=torch.long))
+        off += z
+    dn64 = dn.to(torch.float64)
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        x = th.clone().requires_grad_(True)
+        p = torch.softmax(x, dim=1)
+        n = torch.stack([(p.T @ p[ix].sum(1)).double() for ix in idx])
+        (n * dn64).sum().backward()
+        passes += 1
+        dt = time.perf_counter() - t0
+        if dt >= target_secs or passes >= 200:
+            break
+    return {"value": passes / dt, "unit": "passes/s", "cores": torch.get_num_threads(),
+            "kind": "port", "host": _host_cpu(),
+            "sample": f"{passes} forward + backward passes of the same problem in PyTorch "
+                      f"(float32, CPU), {dt:.1f} s"}
 
 
 def main():
