@@ -157,7 +157,8 @@ def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0=None,
         "kind": kind,
         "sample": f"local search I (T={T}, depth={D}) on {n_lat} {start} "
                   f"{spec.latt_typ} {spec.cell_dim}^3 lattices, "
-                  f"{procs} processes, {st['evals']} evals in {wall:.1f} s",
+                  f"{procs} processes, {st['evals']} evals in {wall:.1f} s wall "
+                  f"(about {wall * procs:.0f} core-seconds)",
     }
 
 
