@@ -339,8 +339,11 @@ def relaxation(a):
     dev = torch.device("cuda", 0)
     g = torch.Generator(device="cpu").manual_seed(wid)
     th = torch.randn((N, K), generator=g, dtype=torch.float32).to(dev)
-    p = torch.empty_like(th)
-    q = torch.empty((S, N, K), dtype=torch.float32, device=dev)
+    from pyhea_amd.relax import relax_cols
+
+    kp = relax_cols(K)
+    p = torch.empty((N, kp), dtype=torch.float32, device=dev)
+    q = torch.empty((S, N, kp), dtype=torch.float32, device=dev)
     n = torch.empty((S, K, K), dtype=torch.float64, device=dev)
     grad = torch.empty_like(th)
     st = torch.cuda.current_stream()

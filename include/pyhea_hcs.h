@@ -203,12 +203,14 @@ int hea_batch_destroy(hea_batch b);
  * only discrete lattices).  With p = softmax(logits) per site, the soft pair
  * counts n[s][a][b] = sum_i sum_{j in N_s(i)} p[i][a] p[j][b] equal the integer
  * pair counts when p is one-hot.  All pointers are device memory; dtype 0 =
- * float32, 1 = float64 for logits, p [N][K], q [S][N][K] (neighbour sums
- * kept for the backward) and grad [N][K]; n and dn are float64 [S][K][K].
- * Asynchronous on `stream`.  Calls on one problem must not overlap.  The
- * first hea_relax_forward or hea_site_gain on a problem allocates its work
- * buffers (about N * (4 * row ints + 64) bytes) and waits for `stream` once,
- * so it must not be captured into a graph. */
+ * float32, 1 = float64 for logits [N][K], grad [N][K], and the occupancies
+ * p [N][KP] and neighbour sums q [S][N][KP] that the forward writes for the
+ * backward, rows padded with zeros to KP = 4 columns for K <= 4, else 8 (p
+ * and q 16-byte aligned); n and dn are float64 [S][K][K].  Asynchronous on
+ * `stream`.  Calls on one problem must not overlap.  The first
+ * hea_relax_forward or hea_site_gain on a problem allocates its work buffers
+ * (about 4 * N * row ints bytes) and waits for `stream` once, so it must not
+ * be captured into a graph. */
 int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
                       void* d_n, void* stream);
 /* Neighbour-gather adjoint: d loss / d logits from dn = d loss / d n. */

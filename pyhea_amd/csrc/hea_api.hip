@@ -286,9 +286,9 @@ struct hea_problem_s {
   DevBuf<int32_t> d_counts, d_perm;
   DevBuf<int64_t> d_stats;
   DevBuf<uint32_t> d_iter;
-  // relaxation: per-block pair-count partials, occupancies padded to G
-  // columns, and the slot-major copy of the neighbour table
-  DevBuf<double> d_relax_part, d_relax_pp;
+  // relaxation: per-workgroup pair-count partials and the slot-major copy
+  // of the neighbour table
+  DevBuf<double> d_relax_part;
   DevBuf<int32_t> d_relax_nbrT;
 };
 
@@ -1446,8 +1446,8 @@ int hea_batch_destroy(hea_batch b) {
 
 // ------------------------------------------------------------- relaxation --
 hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
-                                  int dtype, const void* th, void* p, void* pp, void* Q,
-                                  double* part, double* n, hipStream_t st);
+                                  int dtype, const void* th, void* p, void* Q, double* part,
+                                  double* n, hipStream_t st);
 hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
                                    const void* p, const void* Q, const double* dn, void* grad,
                                    hipStream_t st);
@@ -1464,7 +1464,6 @@ static int relax_prepare(hea_problem p, hipStream_t st) {
   if (p->d_relax_nbrT.p) return HEA_OK;
   const HeaConst& c = p->hc;
   HEA_HIP(p->d_relax_part.alloc(hea_relax_part_size(c.N, c.K, c.S)));
-  HEA_HIP(p->d_relax_pp.alloc((size_t)c.N * 8));
   DevBuf<int32_t> t;
   HEA_HIP(t.alloc((size_t)c.N * c.stride));
   HEA_HIP(hea_relax_transpose_nbr(p->d_nbr.p, c.N, c.stride, t.p, st));
@@ -1476,17 +1475,20 @@ static int relax_prepare(hea_problem p, hipStream_t st) {
 
 extern "C" {
 
+// Padded p and q rows are read and written as 16-byte vectors.
+static bool aligned16(const void* a) { return (reinterpret_cast<uintptr_t>(a) & 15) == 0; }
+
 int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
                       void* d_n, void* stream) {
   if (!p || !d_logits || !d_p || !d_q || !d_n || (dtype != 0 && dtype != 1))
     return fail(HEA_EINVAL, "bad arguments");
+  if (!aligned16(d_p) || !aligned16(d_q)) return fail(HEA_EINVAL, "p and q must be 16-byte aligned");
   HEA_HIP(hipSetDevice(p->device));
   const HeaConst& c = p->hc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (int rc = relax_prepare(p, st)) return rc;
-  HEA_HIP(hea_relax_forward_impl(p->d_const.p, c, p->d_relax_nbrT.p, dtype, d_logits, d_p,
-                                 p->d_relax_pp.p, d_q, p->d_relax_part.p,
-                                 static_cast<double*>(d_n), st));
+  HEA_HIP(hea_relax_forward_impl(p->d_const.p, c, p->d_relax_nbrT.p, dtype, d_logits, d_p, d_q,
+                                 p->d_relax_part.p, static_cast<double*>(d_n), st));
   return HEA_OK;
 }
 
@@ -1494,6 +1496,7 @@ int hea_relax_backward(hea_problem p, int32_t dtype, const void* d_p, const void
                        const void* d_dn, void* d_grad, void* stream) {
   if (!p || !d_p || !d_q || !d_dn || !d_grad || (dtype != 0 && dtype != 1))
     return fail(HEA_EINVAL, "bad arguments");
+  if (!aligned16(d_p) || !aligned16(d_q)) return fail(HEA_EINVAL, "p and q must be 16-byte aligned");
   HEA_HIP(hipSetDevice(p->device));
   HEA_HIP(hea_relax_backward_impl(p->d_const.p, p->hc, dtype, d_p, d_q,
                                   static_cast<const double*>(d_dn), d_grad,

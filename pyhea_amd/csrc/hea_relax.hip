@@ -13,11 +13,12 @@
 // then n_s = P^T Q_s, reduced deterministically in FP64 (a fixed-order sum
 // per block of sites, then the block partials in a fixed-shape tree).
 //
-// Forward layout: the softmax writes p padded to KP columns (KP = 4 for
-// K <= 4, else 8); the gather gives each site KP / 4 lanes of four species
-// each, so a neighbour's row is one 4-element vector load per lane (16 bytes
-// in float32), and reads the neighbour indices from a slot-major copy of the
-// table (nbrT[column][site]), one contiguous run per wave.
+// Layout: p [N][KP] and Q [S][N][KP] have rows padded with zeros to KP
+// columns (KP = 4 for K <= 4, else 8).  The gather gives each site KP / 4
+// lanes of four species each, so a neighbour's row is one 4-element vector
+// load per lane (16 bytes in float32), and reads the neighbour indices from
+// a slot-major copy of the table (nbrT[column][site]), one contiguous run
+// per wave; the adjoint reads whole p and Q rows the same way.
 // Backward (neighbour-gather adjoint): the neighbour relation is symmetric,
 // so dn[s][a][b]/dp[k][c] = [a=c] Q_s[k][b] + [b=c] Q_s[k][a] and
 //     g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q_s[k][b],
@@ -66,12 +67,11 @@ __global__ void k_transpose_nbr(const int32_t* __restrict__ nbr, int N, int stri
   nbrT[t] = nbr[(size_t)i * stride + c];
 }
 
-// p[i][b] = softmax(theta[i])[b], G lanes per site (one per column);
-// pp[i][G] is the same row padded with zeros.
+// p[i][b] = softmax(theta[i])[b] for b < K, 0 up to b < G: G lanes per
+// site, one per column.
 template <typename T, int G>
 __global__ __launch_bounds__(kRelaxThreads) void k_softmax(const T* __restrict__ th,
-                                                           T* __restrict__ p, T* __restrict__ pp,
-                                                           int N, int K) {
+                                                           T* __restrict__ p, int N, int K) {
   const int t = blockIdx.x * kRelaxThreads + threadIdx.x;
   const int i = t / G, b = t % G;
   if (i >= N) return;  // whole G-lane groups leave together
@@ -80,9 +80,7 @@ __global__ __launch_bounds__(kRelaxThreads) void k_softmax(const T* __restrict__
   const double m = group_max<G>(x);
   const double e = on ? exp(x - m) : 0.0;
   const double z = group_sum<G>(e);
-  const T v = on ? (T)(e / z) : (T)0;
-  if (on) p[(size_t)i * K + b] = v;
-  pp[(size_t)i * G + b] = v;
+  p[(size_t)i * G + b] = on ? (T)(e / z) : (T)0;
 }
 
 template <typename T, int V>
@@ -94,7 +92,7 @@ struct alignas(V * sizeof(T)) VecT {
 // b*V .. b*V+V-1 over i's shell-s neighbours, in list order; then the
 // block's share of n[s][a][c] = sum_i p[i][a] Q[s][i][c] from LDS: four
 // interleaved site subsets per entry, added in a fixed order.  Rows of pp
-// have G*V columns; part is [S K K][nb].
+// and Q have G*V columns; part is [S K K][nb].
 template <typename T, int G, int V>
 __global__ __launch_bounds__(kGatherThreads) void k_gather_pairs(
     const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbrT, const T* __restrict__ pp,
@@ -142,13 +140,13 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_pairs(
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[v] += (double)vk.x[v];
       }
+      Vec qv;
+#pragma unroll
+      for (int v = 0; v < V; ++v) qv.x[v] = (T)acc[v];
       T* myq = sq + s * SB * KP + il * KP + b * V;
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const T q = (T)acc[v];
-        myq[v] = q;
-        if (b * V + v < K) Q[((size_t)s * N + i) * K + b * V + v] = q;
-      }
+      for (int v = 0; v < V; ++v) myq[v] = qv.x[v];
+      *reinterpret_cast<Vec*>(Q + ((size_t)s * N + i) * KP + b * V) = qv;
     }
   } else {
 #pragma unroll
@@ -191,51 +189,50 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const double* __restrict__ 
 
 // grad[k][c] = p[k][c] (g[k][c] - sum_d p[k][d] g[k][d]),
 // g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; one lane per
-// site, whose p and Q rows are loaded before the workgroup builds the
-// symmetrised table H in LDS, so the loads overlap that step.
-template <typename T>
+// site, whose padded p and Q rows (KP columns) are loaded as vectors before
+// the workgroup builds the symmetrised table H in LDS, so the loads overlap
+// that step.  grad is [N][K].
+template <typename T, int KP>
 __global__ __launch_bounds__(256) void k_adjoint(const HeaConst* __restrict__ pc,
                                                  const T* __restrict__ p,
                                                  const T* __restrict__ Q,
                                                  const double* __restrict__ dn,
                                                  T* __restrict__ grad) {
+  using Row = VecT<T, KP>;
   const HeaConst& P = *pc;
   const int K = P.K, S = P.S, N = P.N;
   __shared__ double H[HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
   const int k = blockIdx.x * 256 + threadIdx.x;
   const bool live = k < N;
-  T q[HEA_MAX_S][HEA_MAX_K], pr[HEA_MAX_K];
+  Row q[HEA_MAX_S], pr;
 #pragma unroll
   for (int s = 0; s < HEA_MAX_S; ++s)
-#pragma unroll
-    for (int b = 0; b < HEA_MAX_K; ++b)
-      q[s][b] = (live && s < S && b < K) ? Q[((size_t)s * N + k) * K + b] : (T)0;
-#pragma unroll
-  for (int b = 0; b < HEA_MAX_K; ++b) pr[b] = (live && b < K) ? p[(size_t)k * K + b] : (T)0;
+    if (live && s < S) q[s] = *reinterpret_cast<const Row*>(Q + ((size_t)s * N + k) * KP);
+  if (live) pr = *reinterpret_cast<const Row*>(p + (size_t)k * KP);
   for (int e = threadIdx.x; e < S * K * K; e += blockDim.x) {
     const int s = e / (K * K), c = (e / K) % K, b = e % K;
     H[e] = dn[(s * K + c) * K + b] + dn[(s * K + b) * K + c];
   }
   __syncthreads();
   if (!live) return;
-  double g[HEA_MAX_K];
+  double g[KP];
 #pragma unroll
-  for (int c = 0; c < HEA_MAX_K; ++c) {
+  for (int c = 0; c < KP; ++c) {
     double a = 0.0;
 #pragma unroll
     for (int s = 0; s < HEA_MAX_S; ++s)
 #pragma unroll
-      for (int b = 0; b < HEA_MAX_K; ++b)
-        if (c < K && s < S && b < K) a += H[(s * K + c) * K + b] * (double)q[s][b];
+      for (int b = 0; b < KP; ++b)
+        if (c < K && s < S && b < K) a += H[(s * K + c) * K + b] * (double)q[s].x[b];
     g[c] = a;
   }
   double dot = 0.0;
 #pragma unroll
-  for (int c = 0; c < HEA_MAX_K; ++c)
-    if (c < K) dot += (double)pr[c] * g[c];
+  for (int c = 0; c < KP; ++c)
+    if (c < K) dot += (double)pr.x[c] * g[c];
 #pragma unroll
-  for (int c = 0; c < HEA_MAX_K; ++c)
-    if (c < K) grad[(size_t)k * K + c] = (T)((double)pr[c] * (g[c] - dot));
+  for (int c = 0; c < KP; ++c)
+    if (c < K) grad[(size_t)k * K + c] = (T)((double)pr.x[c] * (g[c] - dot));
 }
 
 // u[l][k][b] = g[k][b] - g[k][a_k] at the one-hot lattice l (see file header).
@@ -278,30 +275,31 @@ inline int relax_gather_blocks(int N, int K) {
 
 template <typename T, int KP>
 hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
-                           const void* th, void* p, void* pp, void* Q, double* part, double* n,
+                           const void* th, void* p, void* Q, double* part, double* n,
                            hipStream_t st) {
   constexpr int SBS = kRelaxThreads / KP;
   hipLaunchKernelGGL((k_softmax<T, KP>), dim3((hc.N + SBS - 1) / SBS), dim3(kRelaxThreads), 0,
-                     st, static_cast<const T*>(th), static_cast<T*>(p), static_cast<T*>(pp), hc.N,
-                     hc.K);
+                     st, static_cast<const T*>(th), static_cast<T*>(p), hc.N, hc.K);
   const int nb = relax_gather_blocks(hc.N, hc.K);
   hipLaunchKernelGGL((k_gather_pairs<T, KP / kGatherVec, kGatherVec>), dim3(nb),
-                     dim3(kGatherThreads), 0, st, dc, nbrT, static_cast<const T*>(pp),
+                     dim3(kGatherThreads), 0, st, dc, nbrT, static_cast<const T*>(p),
                      static_cast<T*>(Q), part);
   hipLaunchKernelGGL(k_pair_reduce, dim3(hc.S * hc.K * hc.K), dim3(256), 0, st, part, nb, n);
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, int KP>
 hipError_t relax_backward_t(const HeaConst* dc, const HeaConst& hc, const void* p, const void* Q,
                             const double* dn, void* grad, hipStream_t st) {
-  hipLaunchKernelGGL((k_adjoint<T>), dim3((hc.N + 255) / 256), dim3(256), 0, st, dc,
+  hipLaunchKernelGGL((k_adjoint<T, KP>), dim3((hc.N + 255) / 256), dim3(256), 0, st, dc,
                      static_cast<const T*>(p), static_cast<const T*>(Q), dn,
                      static_cast<T*>(grad));
   return hipGetLastError();
 }
 
 }  // namespace
+
+int hea_relax_cols(int K) { return relax_cols(K); }
 
 size_t hea_relax_part_size(int n_sites, int K, int S) {
   return (size_t)relax_gather_blocks(n_sites, K) * S * K * K;
@@ -316,21 +314,25 @@ hipError_t hea_relax_transpose_nbr(const int32_t* nbr, int N, int stride, int32_
 }
 
 hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
-                                  int dtype, const void* th, void* p, void* pp, void* Q,
-                                  double* part, double* n, hipStream_t st) {
+                                  int dtype, const void* th, void* p, void* Q, double* part,
+                                  double* n, hipStream_t st) {
   const bool g4 = relax_cols(hc.K) == 4;
   if (dtype == 1)
-    return g4 ? relax_forward_g<double, 4>(dc, hc, nbrT, th, p, pp, Q, part, n, st)
-              : relax_forward_g<double, 8>(dc, hc, nbrT, th, p, pp, Q, part, n, st);
-  return g4 ? relax_forward_g<float, 4>(dc, hc, nbrT, th, p, pp, Q, part, n, st)
-            : relax_forward_g<float, 8>(dc, hc, nbrT, th, p, pp, Q, part, n, st);
+    return g4 ? relax_forward_g<double, 4>(dc, hc, nbrT, th, p, Q, part, n, st)
+              : relax_forward_g<double, 8>(dc, hc, nbrT, th, p, Q, part, n, st);
+  return g4 ? relax_forward_g<float, 4>(dc, hc, nbrT, th, p, Q, part, n, st)
+            : relax_forward_g<float, 8>(dc, hc, nbrT, th, p, Q, part, n, st);
 }
 
 hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
                                    const void* p, const void* Q, const double* dn, void* grad,
                                    hipStream_t st) {
-  return dtype == 1 ? relax_backward_t<double>(dc, hc, p, Q, dn, grad, st)
-                    : relax_backward_t<float>(dc, hc, p, Q, dn, grad, st);
+  const bool g4 = relax_cols(hc.K) == 4;
+  if (dtype == 1)
+    return g4 ? relax_backward_t<double, 4>(dc, hc, p, Q, dn, grad, st)
+              : relax_backward_t<double, 8>(dc, hc, p, Q, dn, grad, st);
+  return g4 ? relax_backward_t<float, 4>(dc, hc, p, Q, dn, grad, st)
+            : relax_backward_t<float, 8>(dc, hc, p, Q, dn, grad, st);
 }
 
 hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,

@@ -3,8 +3,10 @@
 ``SoftPairCounts`` maps site logits [N, K] to soft pair counts n [S, K, K]
 (float64) with the HIP forward (softmax, neighbour gather, deterministic
 FP64 reduction) and the hand-written HIP backward (neighbour-gather adjoint)
-of ``hea_relax_forward`` / ``hea_relax_backward``.  ``soft_fitness_sq``
-builds the relaxed objective on top with ordinary torch ops:
+of ``hea_relax_forward`` / ``hea_relax_backward``; the occupancies and
+neighbour sums it keeps for the backward have rows padded to
+``relax_cols(K)`` columns.  ``soft_fitness_sq`` builds the relaxed
+objective on top with ordinary torch ops:
 
     gamma = n / (Z_s N),  alpha = 1 - gamma / (c_a c_b),
     F^2   = sum_s w_s sum_ab (alpha_sab - t_sab)^2
@@ -31,6 +33,12 @@ def _dtype_code(t: torch.Tensor) -> int:
     raise TypeError("logits must be float32 or float64")
 
 
+def relax_cols(K: int) -> int:
+    """Columns of the padded occupancy rows p [N][KP] and neighbour sums
+    q [S][N][KP] (zeros beyond K)."""
+    return 4 if K <= 4 else 8
+
+
 def _stream() -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -44,8 +52,9 @@ class SoftPairCounts(torch.autograd.Function):
         N, K, S = problem.n_sites, problem.K, problem.S
         if tuple(lg.shape) != (N, K):
             raise ValueError(f"logits must be [{N}, {K}]")
-        p = torch.empty_like(lg)
-        q = torch.empty((S, N, K), dtype=lg.dtype, device=lg.device)
+        kp = relax_cols(K)
+        p = torch.empty((N, kp), dtype=lg.dtype, device=lg.device)
+        q = torch.empty((S, N, kp), dtype=lg.dtype, device=lg.device)
         n = torch.empty((S, K, K), dtype=torch.float64, device=lg.device)
         Nat.check(Nat.lib().hea_relax_forward(problem.handle, _dtype_code(lg), lg.data_ptr(),
                                               p.data_ptr(), q.data_ptr(), n.data_ptr(),
@@ -58,7 +67,7 @@ class SoftPairCounts(torch.autograd.Function):
     def backward(ctx, dn: torch.Tensor):
         p, q = ctx.saved_tensors
         dn = dn.to(torch.float64).contiguous()
-        grad = torch.empty_like(p)
+        grad = torch.empty((p.shape[0], ctx.problem.K), dtype=p.dtype, device=p.device)
         Nat.check(Nat.lib().hea_relax_backward(ctx.problem.handle, _dtype_code(p), p.data_ptr(),
                                                q.data_ptr(), dn.data_ptr(), grad.data_ptr(),
                                                _stream()))

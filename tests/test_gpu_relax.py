@@ -227,3 +227,21 @@ def test_side_stream_gives_same_bits(gpu):
         torch.cuda.synchronize()
         out.append((n.detach().clone(), th.grad.clone()))
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
+def test_misaligned_buffers_are_rejected(gpu):
+    """p and q rows are moved as 16-byte vectors: a q that is not 16-byte
+    aligned is an argument error, raised before any kernel runs."""
+    from pyhea_amd import _native as Nat
+    from pyhea_amd.relax import relax_cols
+
+    spec = _spec("FCC", 3, 5, 3)
+    prob = Problem.from_spec(spec)
+    N, kp = spec.n_sites, relax_cols(5)
+    th = torch.randn(N, 5, device="cuda")
+    p = torch.empty((N, kp), device="cuda")
+    q = torch.empty(3 * N * kp + 4, device="cuda")
+    n = torch.empty((3, 5, 5), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="aligned"):
+        Nat.check(Nat.lib().hea_relax_forward(prob.handle, 0, th.data_ptr(), p.data_ptr(),
+                                              q.data_ptr() + 4, n.data_ptr(), None))
