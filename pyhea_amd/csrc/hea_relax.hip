@@ -34,9 +34,9 @@
 
 namespace {
 
-constexpr int kRelaxThreads = 1024;  // softmax lanes per workgroup (KP lanes per site)
-constexpr int kGatherThreads = 512;  // gather lanes per workgroup
-constexpr int kGatherVec = 4;        // species per gather lane
+constexpr int kSoftmaxThreads = 512;  // softmax lanes per workgroup
+constexpr int kGatherThreads = 512;   // gather lanes per workgroup
+constexpr int kGatherVec = 4;         // species per gather lane
 
 // Workgroups are dispatched round-robin over the eight XCDs; this bijection
 // of [0, nb) gives each XCD one contiguous range of logical blocks, so the
@@ -44,18 +44,6 @@ constexpr int kGatherVec = 4;        // species per gather lane
 __device__ __forceinline__ int xcd_block(int w, int nb) {
   const int q = nb >> 3, r = nb & 7, x = w & 7, i = w >> 3;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
-template <int G>
-__device__ __forceinline__ double group_max(double v) {
-  for (int o = 1; o < G; o <<= 1) v = fmax(v, __shfl_xor(v, o, G));
-  return v;
-}
-
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, G);
-  return v;
 }
 
 // nbrT[c][i] = nbr[i][c]: the slot-major copy of the neighbour table.
@@ -67,26 +55,41 @@ __global__ void k_transpose_nbr(const int32_t* __restrict__ nbr, int N, int stri
   nbrT[t] = nbr[(size_t)i * stride + c];
 }
 
-// p[i][b] = softmax(theta[i])[b] for b < K, 0 up to b < G: G lanes per
-// site, one per column.
-template <typename T, int G>
-__global__ __launch_bounds__(kRelaxThreads) void k_softmax(const T* __restrict__ th,
-                                                           T* __restrict__ p, int N, int K) {
-  const int t = blockIdx.x * kRelaxThreads + threadIdx.x;
-  const int i = t / G, b = t % G;
-  if (i >= N) return;  // whole G-lane groups leave together
-  const bool on = b < K;
-  const double x = on ? (double)th[(size_t)i * K + b] : -HUGE_VAL;
-  const double m = group_max<G>(x);
-  const double e = on ? exp(x - m) : 0.0;
-  const double z = group_sum<G>(e);
-  p[(size_t)i * G + b] = on ? (T)(e / z) : (T)0;
-}
-
 template <typename T, int V>
 struct alignas(V * sizeof(T)) VecT {
   T x[V];
 };
+
+// p[i][c] = softmax(theta[i])[c] for c < K, 0 up to c < KP: KP / 4 lanes
+// per site, four columns each, written as one vector.
+template <typename T, int KP>
+__global__ __launch_bounds__(kSoftmaxThreads) void k_softmax(const T* __restrict__ th,
+                                                             T* __restrict__ p, int N, int K) {
+  constexpr int GL = KP / 4;
+  using Vec = VecT<T, 4>;
+  const int t = blockIdx.x * kSoftmaxThreads + threadIdx.x;
+  const int i = t / GL, b = t % GL;
+  if (i >= N) return;  // a site's lanes leave together
+  double x[4], m = -HUGE_VAL;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int c = b * 4 + v;
+    x[v] = c < K ? (double)th[(size_t)i * K + c] : -HUGE_VAL;
+    m = fmax(m, x[v]);
+  }
+  if (GL > 1) m = fmax(m, __shfl_xor(m, 1, GL));
+  double z = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    x[v] = b * 4 + v < K ? exp(x[v] - m) : 0.0;
+    z += x[v];
+  }
+  if (GL > 1) z += __shfl_xor(z, 1, GL);
+  Vec out;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out.x[v] = (T)(x[v] / z);
+  *reinterpret_cast<Vec*>(p + (size_t)i * KP + b * 4) = out;
+}
 
 // Q[s][i][c] for one block of SB sites: lane b of site i sums species
 // b*V .. b*V+V-1 over i's shell-s neighbours, in list order; then the
@@ -277,8 +280,8 @@ template <typename T, int KP>
 hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                            const void* th, void* p, void* Q, double* part, double* n,
                            hipStream_t st) {
-  constexpr int SBS = kRelaxThreads / KP;
-  hipLaunchKernelGGL((k_softmax<T, KP>), dim3((hc.N + SBS - 1) / SBS), dim3(kRelaxThreads), 0,
+  constexpr int SBS = kSoftmaxThreads / (KP / 4);
+  hipLaunchKernelGGL((k_softmax<T, KP>), dim3((hc.N + SBS - 1) / SBS), dim3(kSoftmaxThreads), 0,
                      st, static_cast<const T*>(th), static_cast<T*>(p), hc.N, hc.K);
   const int nb = relax_gather_blocks(hc.N, hc.K);
   hipLaunchKernelGGL((k_gather_pairs<T, KP / kGatherVec, kGatherVec>), dim3(nb),
