@@ -18,7 +18,7 @@
 // lanes of four species each, so a neighbour's row is one 4-element vector
 // load per lane (16 bytes in float32), and reads the neighbour indices from
 // a slot-major copy of the table (nbrT[column][site]), one contiguous run
-// per wave; the adjoint reads whole p and Q rows the same way.
+// per wave; the adjoint reads Q rows and p columns as vectors too.
 // Backward (neighbour-gather adjoint): the neighbour relation is symmetric,
 // so dn[s][a][b]/dp[k][c] = [a=c] Q_s[k][b] + [b=c] Q_s[k][a] and
 //     g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q_s[k][b],
@@ -191,51 +191,57 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const double* __restrict__ 
 }
 
 // grad[k][c] = p[k][c] (g[k][c] - sum_d p[k][d] g[k][d]),
-// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; one lane per
-// site, whose padded p and Q rows (KP columns) are loaded as vectors before
-// the workgroup builds the symmetrised table H in LDS, so the loads overlap
-// that step.  grad is [N][K].
+// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; KP / 4 lanes
+// per site, four columns c each.  Every lane loads the site's whole padded
+// Q rows and its own four p columns as vectors before the workgroup builds
+// the symmetrised table H in LDS, so the loads overlap that step; the two
+// halves of <p, g> meet through one shuffle.  grad is [N][K].
 template <typename T, int KP>
 __global__ __launch_bounds__(256) void k_adjoint(const HeaConst* __restrict__ pc,
                                                  const T* __restrict__ p,
                                                  const T* __restrict__ Q,
                                                  const double* __restrict__ dn,
                                                  T* __restrict__ grad) {
+  constexpr int GL = KP / 4;
   using Row = VecT<T, KP>;
+  using Vec = VecT<T, 4>;
   const HeaConst& P = *pc;
   const int K = P.K, S = P.S, N = P.N;
   __shared__ double H[HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int k = t / GL, b = t % GL;
   const bool live = k < N;
-  Row q[HEA_MAX_S], pr;
+  Row q[HEA_MAX_S];
+  Vec pr;
 #pragma unroll
   for (int s = 0; s < HEA_MAX_S; ++s)
     if (live && s < S) q[s] = *reinterpret_cast<const Row*>(Q + ((size_t)s * N + k) * KP);
-  if (live) pr = *reinterpret_cast<const Row*>(p + (size_t)k * KP);
+  if (live) pr = *reinterpret_cast<const Vec*>(p + (size_t)k * KP + b * 4);
   for (int e = threadIdx.x; e < S * K * K; e += blockDim.x) {
-    const int s = e / (K * K), c = (e / K) % K, b = e % K;
-    H[e] = dn[(s * K + c) * K + b] + dn[(s * K + b) * K + c];
+    const int s = e / (K * K), c = (e / K) % K, bb = e % K;
+    H[e] = dn[(s * K + c) * K + bb] + dn[(s * K + bb) * K + c];
   }
   __syncthreads();
   if (!live) return;
-  double g[KP];
+  double g[4], dot = 0.0;
 #pragma unroll
-  for (int c = 0; c < KP; ++c) {
+  for (int v = 0; v < 4; ++v) {
+    const int c = b * 4 + v;
     double a = 0.0;
 #pragma unroll
     for (int s = 0; s < HEA_MAX_S; ++s)
 #pragma unroll
-      for (int b = 0; b < KP; ++b)
-        if (c < K && s < S && b < K) a += H[(s * K + c) * K + b] * (double)q[s].x[b];
-    g[c] = a;
+      for (int bb = 0; bb < KP; ++bb)
+        if (c < K && s < S && bb < K) a += H[(s * K + c) * K + bb] * (double)q[s].x[bb];
+    g[v] = a;
+    if (c < K) dot += (double)pr.x[v] * a;
   }
-  double dot = 0.0;
+  if (GL > 1) dot += __shfl_xor(dot, 1, GL);
 #pragma unroll
-  for (int c = 0; c < KP; ++c)
-    if (c < K) dot += (double)pr.x[c] * g[c];
-#pragma unroll
-  for (int c = 0; c < KP; ++c)
-    if (c < K) grad[(size_t)k * K + c] = (T)((double)pr.x[c] * (g[c] - dot));
+  for (int v = 0; v < 4; ++v) {
+    const int c = b * 4 + v;
+    if (c < K) grad[(size_t)k * K + c] = (T)((double)pr.x[v] * (g[v] - dot));
+  }
 }
 
 // u[l][k][b] = g[k][b] - g[k][a_k] at the one-hot lattice l (see file header).
@@ -294,7 +300,8 @@ hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t
 template <typename T, int KP>
 hipError_t relax_backward_t(const HeaConst* dc, const HeaConst& hc, const void* p, const void* Q,
                             const double* dn, void* grad, hipStream_t st) {
-  hipLaunchKernelGGL((k_adjoint<T, KP>), dim3((hc.N + 255) / 256), dim3(256), 0, st, dc,
+  const int lanes = hc.N * (KP / 4);
+  hipLaunchKernelGGL((k_adjoint<T, KP>), dim3((lanes + 255) / 256), dim3(256), 0, st, dc,
                      static_cast<const T*>(p), static_cast<const T*>(Q), dn,
                      static_cast<T*>(grad));
   return hipGetLastError();
