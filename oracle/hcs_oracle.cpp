@@ -205,12 +205,16 @@ double local_mc_one(const Problem& P, uint8_t* lat, double fit, int task,
   return prev_fit;
 }
 
-// Stable order by (fitness, position); reports whether two keys were equal
-// (the reference's std::sort is unstable there, accelerate.cpp:47).
-int stable_order(const double* f, int n, std::vector<int>& idx) {
+// The reference's population order (accelerate.cpp:37-64): std::sort of the
+// index vector 0..n-1 by fitness.  std::sort is not stable, so on equal keys
+// the order is whatever the standard library's introsort leaves; calling the
+// same library routine on the same int indices with the same comparator
+// reproduces it (libstdc++ of GCC 11.4 here and in the reference build).
+// Reports whether two keys were equal.
+int sort_order(const double* f, int n, std::vector<int>& idx) {
   idx.resize(n);
   std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return f[a] < f[b]; });
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return f[a] < f[b]; });
   int ties = 0;
   for (int i = 1; i < n; ++i)
     if (f[idx[i]] == f[idx[i - 1]]) ties = 1;
@@ -314,7 +318,7 @@ void oracle_incremental_chain(int N, int K, int S, const int32_t* Z, const int32
 }
 
 // accelerate.cpp:601-694 on a population of L lattices (slot = position),
-// stream (seed, iter, phase, slot).  Ends with the stable sort
+// stream (seed, iter, phase, slot).  Ends with the population sort
 // (accelerate.cpp:693).  stats: [evals, rounds, accepts, draws, ties];
 // per_slot (optional) [L][4]: rounds, accepts, draws, evals before the sort.
 int oracle_local_mc(int N, int K, int S, const int32_t* Z, const int32_t* nbr,
@@ -343,7 +347,7 @@ int oracle_local_mc(int N, int K, int S, const int32_t* Z, const int32_t* nbr,
   int ties = 0;
   if (do_sort) {
     std::vector<int> idx;
-    ties = stable_order(fit, L, idx);
+    ties = sort_order(fit, L, idx);
     reorder(lats, fit, L, N, idx);
   }
   if (stats) {
@@ -411,7 +415,7 @@ int oracle_run_hcs(int N, int K, int S, const int32_t* Z, const int32_t* nbr,
       memcpy(cf.data(), fit.data(), sizeof(double) * L);
       memcpy(cf.data() + L, nf.data(), sizeof(double) * L);
       std::vector<int> idx;
-      tot[4] |= stable_order(cf.data(), 2 * L, idx);
+      tot[4] |= sort_order(cf.data(), 2 * L, idx);
       for (int r = 0; r < L; ++r) {
         memcpy(&lat[(size_t)r * N], &cl[(size_t)idx[r] * N], N);
         fit[r] = cf[idx[r]];

@@ -1,8 +1,9 @@
 // TEST INFRASTRUCTURE ONLY: drives the reference CPU engine itself.
 //
-// Built by oracle/build_ref.sh into oracle/_ref/ref_hcs from the reference
-// source where it lies (pyhea/cpp/accelerate.cpp is #included by path at
-// build time; none of its text is kept in this repository).  The binary is a
+// Built by oracle/Makefile (target `ref`) into oracle/_ref/ref_hcs from the
+// reference source where it lies (pyhea/cpp/accelerate.cpp is #included by
+// path at build time; neither its text nor the binary is kept in this
+// repository's history: oracle/_ref/ is git-ignored).  The binary is a
 // checker and the "reference" CPU baseline; product code never runs it.
 //
 // Two inputs of the reference are not reproducible, so the harness injects

@@ -54,13 +54,27 @@ def _run(cmd: list[str], cwd: str | None = None) -> None:
         raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}\n{res.stdout}")
 
 
-def build_hip(verbose: bool = False) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """True when obj is older than its source or any header of csrc/ or include/."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(d, f) for d in (CSRC, os.path.join(ROOT, "include"))
+                    for f in os.listdir(d) if f.endswith(".h")]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_hip(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every HIP source (incremental=True: only those whose object is
+    out of date) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
     sources = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
     def compile_one(src: str) -> str:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            return obj
         _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         if verbose:
             print(f"[build] {src}", flush=True)
@@ -126,12 +140,14 @@ def build_oracle(with_reference: bool | None = None) -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), f"REF_ROOT={ref_root}", *targets])
 
 
-def build_all(verbose: bool = False) -> None:
-    build_hip(verbose)
+def build_all(verbose: bool = False, incremental: bool = False) -> None:
+    build_hip(verbose, incremental)
     build_bind(verbose)
     build_profile_driver(verbose)
 
 
 if __name__ == "__main__":
-    build_all(verbose=True)
+    import sys
+
+    build_all(verbose=True, incremental="--incremental" in sys.argv)
     build_oracle()

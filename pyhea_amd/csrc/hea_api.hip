@@ -20,6 +20,7 @@
 
 #include "../../include/pyhea_hcs.h"
 #include "hea_internal.h"
+#include "hea_sort.h"
 
 namespace {
 
@@ -200,21 +201,59 @@ __global__ void k_fitness(const HeaConst* __restrict__ pc, const int32_t* __rest
   fit[l] = sqrt(error);
 }
 
-// Stable rank by (fitness, position) over fa[0..na) ++ fb[0..nb); writes
-// perm[rank] = position for ranks below `keep`.  Fitness values are finite
-// and >= 0, so comparing bit patterns as unsigned integers orders them.
+// Rank by (fitness, position) over fa[0..na) ++ fb[0..nb); writes
+// perm[rank] = position for ranks below `keep`.  This is the reference's
+// order (std::sort of the positions by fitness, accelerate.cpp:37-64) as
+// long as no two kept keys are equal; a kept key that equals another raises
+// *tie, and k_tie_sort then redoes the order with the reference's algorithm.
 __global__ void k_rank(const double* __restrict__ fa, int na, const double* __restrict__ fb,
-                       int nb, int keep, int32_t* __restrict__ perm) {
+                       int nb, int keep, int32_t* __restrict__ perm, uint32_t* __restrict__ tie) {
   const int n = na + nb;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double fi = i < na ? fa[i] : fb[i - na];
   int r = 0;
+  bool eq = false;
   for (int j = 0; j < n; ++j) {
     const double fj = j < na ? fa[j] : fb[j - na];
     r += (fj < fi) || (fj == fi && j < i);
+    eq |= (fj == fi) && (j != i);
   }
-  if (r < keep) perm[r] = i;
+  if (r < keep) {
+    perm[r] = i;
+    if (eq) *tie = 1u;
+  }
+}
+
+// Population order on equal keys: when k_rank saw a tie, one lane sorts the
+// positions 0..n-1 with the reference library's std::sort algorithm
+// (hea_sort.h), in LDS when `lds` holds the keys and positions (12 bytes
+// each), else in place in perm (n entries) reading the keys from HBM; then
+// clears *tie.  Without a tie the kernel returns at once.
+__global__ __launch_bounds__(256) void k_tie_sort(const double* __restrict__ fa, int na,
+                                                  const double* __restrict__ fb, int nb, int keep,
+                                                  int32_t* __restrict__ perm,
+                                                  uint32_t* __restrict__ tie, int lds) {
+  if (__builtin_amdgcn_readfirstlane((int)*tie) == 0) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = na + nb;
+  const bool in_lds = lds > 0;
+  double* key = reinterpret_cast<double*>(smem);
+  int* idx = in_lds ? reinterpret_cast<int*>(smem + sizeof(double) * (size_t)n) : perm;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (in_lds) key[i] = i < na ? fa[i] : fb[i - na];
+    idx[i] = i;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const hea_sort_keys K = in_lds ? hea_sort_keys{key, nullptr, (int64_t)n}
+                                   : hea_sort_keys{fa, fb, (int64_t)na};
+    hea_sort_like_std(idx, n, K);
+  }
+  __syncthreads();
+  if (in_lds)
+    for (int r = threadIdx.x; r < keep; r += blockDim.x) perm[r] = idx[r];
+  if (threadIdx.x == 0) *tie = 0u;
 }
 
 // dst row r = (perm[r] < na ? srcA[perm[r]] : srcB[perm[r]-na]).
@@ -286,6 +325,7 @@ struct hea_problem_s {
   DevBuf<int32_t> d_counts, d_perm;
   DevBuf<int64_t> d_stats;
   DevBuf<uint32_t> d_iter;
+  DevBuf<uint32_t> d_tie;  // tie flag of the population sort (zero between calls)
   // relaxation: per-workgroup pair-count partials and the slot-major copy
   // of the neighbour table
   DevBuf<double> d_relax_part;
@@ -307,6 +347,7 @@ struct hea_session_s {
   DevBuf<int32_t> counts, perm;
   DevBuf<int64_t> stats;
   DevBuf<uint32_t> iter;
+  DevBuf<uint32_t> tie;            // tie flag of the population sorts (zero between sorts)
   DevBuf<unsigned long long> clk;  // [0] start, [1] first fitness <= thr
   int iters_done = 0;
   DevBuf<unsigned long long> hist;  // guided mode, [L][N]
@@ -385,6 +426,10 @@ int ensure_scratch(hea_problem p, int L) {
   const size_t cn = (size_t)L * c.S * c.K * c.K;
   if (p->d_counts.n < cn) HEA_HIP(p->d_counts.alloc(cn));
   if (!p->d_iter.p) HEA_HIP(p->d_iter.alloc(1));
+  if (!p->d_tie.p) {
+    HEA_HIP(p->d_tie.alloc(1));
+    HEA_HIP(hipMemset(p->d_tie.p, 0, sizeof(uint32_t)));
+  }
   return HEA_OK;
 }
 
@@ -414,12 +459,18 @@ int launch_fitness(hea_problem p, const int32_t* counts, double* fit, int L, hip
   return HEA_OK;
 }
 
-// dst = stable order of (fa ++ fb), first `keep` rows.
+// dst = the reference's order of (fa ++ fb), first `keep` rows.  perm holds
+// at least na + nb entries; *tie is zero between calls.
 int launch_merge(const uint8_t* A, const double* fa, int na, const uint8_t* B, const double* fb,
-                 int nb, int keep, int32_t* perm, uint8_t* dst, double* fdst, int N,
-                 hipStream_t st) {
+                 int nb, int keep, int32_t* perm, uint32_t* tie, uint8_t* dst, double* fdst,
+                 int N, hipStream_t st) {
   const int n = na + nb;
-  hipLaunchKernelGGL(k_rank, dim3((n + 255) / 256), dim3(256), 0, st, fa, na, fb, nb, keep, perm);
+  hipLaunchKernelGGL(k_rank, dim3((n + 255) / 256), dim3(256), 0, st, fa, na, fb, nb, keep, perm,
+                     tie);
+  const size_t tl = (size_t)n * (sizeof(double) + sizeof(int));
+  const int lds = tl <= 65536 ? (int)tl : 0;
+  hipLaunchKernelGGL(k_tie_sort, dim3(1), dim3(256), (size_t)lds, st, fa, na, fb, nb, keep, perm,
+                     tie, lds);
   const int per = (N % 16 == 0) ? N / 16 : N;
   const unsigned gx = (unsigned)std::max(1, std::min(64, (per + 255) / 256));
   hipLaunchKernelGGL(k_gather, dim3(gx, keep), dim3(256), 0, st, A, fa, na, B, fb, perm, dst,
@@ -718,8 +769,8 @@ static int local_mc_dev(hea_problem p, int L, uint8_t* lat, double* fit, int tas
                       depth, seed, phase, -1.0, lds, sp_lds, st)))
     return rc;
   if (sort) {
-    if ((rc = launch_merge(lat, fit, L, lat, fit, 0, L, p->d_perm.p, p->d_lat2.p, p->d_fit2.p,
-                           p->hc.N, st)))
+    if ((rc = launch_merge(lat, fit, L, lat, fit, 0, L, p->d_perm.p, p->d_tie.p, p->d_lat2.p,
+                           p->d_fit2.p, p->hc.N, st)))
       return rc;
     HEA_HIP(hipMemcpyAsync(lat, p->d_lat2.p, (size_t)L * p->hc.N, hipMemcpyDeviceToDevice, st));
     HEA_HIP(hipMemcpyAsync(fit, p->d_fit2.p, sizeof(double) * L, hipMemcpyDeviceToDevice, st));
@@ -800,12 +851,12 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
     HEA_HIP(hipEventRecord(hk->ev[1], st));
     HEA_HIP(hipMemcpyAsync(hk->snap + 4 * L, s->stats.p, sb, hipMemcpyDeviceToDevice, st));
   }
-  if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->C.p, s->fC.p,
-                         N, st)))
+  if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->tie.p, s->C.p,
+                         s->fC.p, N, st)))
     return rc;
   // merge old ++ new, keep the best L (accelerate.cpp:747, 305-330)
-  if ((rc = launch_merge(s->A.p, s->fA.p, L, s->C.p, s->fC.p, L, L, s->perm.p, s->B.p, s->fB.p, N,
-                         st)))
+  if ((rc = launch_merge(s->A.p, s->fA.p, L, s->C.p, s->fC.p, L, L, s->perm.p, s->tie.p, s->B.p,
+                         s->fB.p, N, st)))
     return rc;
   // local search II + sort (accelerate.cpp:749, 693)
   if ((rc = launch_recount(p, s->B.p, s->counts.p, L, st))) return rc;
@@ -818,8 +869,8 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
     HEA_HIP(hipEventRecord(hk->ev[3], st));
     HEA_HIP(hipMemcpyAsync(hk->snap + 8 * L, s->stats.p, sb, hipMemcpyDeviceToDevice, st));
   }
-  if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->A.p, s->fA.p,
-                         N, st)))
+  if ((rc = launch_merge(s->B.p, s->fB.p, L, s->B.p, s->fB.p, 0, L, s->perm.p, s->tie.p, s->A.p,
+                         s->fA.p, N, st)))
     return rc;
   hipLaunchKernelGGL(k_iter_add, dim3(1), dim3(1), 0, st, s->iter.p, 1u);
   HEA_HIP(hipGetLastError());
@@ -854,6 +905,8 @@ int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, do
   if (e == hipSuccess) e = s->stats.alloc((size_t)4 * L);
   if (e == hipSuccess) e = s->iter.alloc(1);
   if (e == hipSuccess) e = s->clk.alloc(2);
+  if (e == hipSuccess) e = s->tie.alloc(1);
+  if (e == hipSuccess) e = hipMemset(s->tie.p, 0, sizeof(uint32_t));
   if (e != hipSuccess) {
     delete s;
     return fail(HEA_ENOMEM, std::string("session allocation: ") + hipGetErrorString(e));
@@ -1095,6 +1148,7 @@ struct hea_batch_s {
   DevBuf<int32_t> counts, perm;
   DevBuf<int64_t> stats;
   DevBuf<uint32_t> iter, queue;
+  DevBuf<uint32_t> tie;            // [problems] tie flags of the population sorts
   DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
   DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
   DevBuf<int2> items;
@@ -1138,12 +1192,13 @@ static int batch_enqueue_iteration(hea_batch b) {
       double* fB = b->fB.p + (size_t)q * L;
       double* fC = b->fC.p + (size_t)q * L;
       int32_t* perm = b->perm.p + (size_t)q * 2 * L;
+      uint32_t* tie = b->tie.p + q;
       if (phase == HEA_PHASE_LS1) {
-        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, C, fC, N, st))) return rc;
-        if ((rc = launch_merge(A, fA, L, C, fC, L, L, perm, B, fB, N, st))) return rc;
+        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, tie, C, fC, N, st))) return rc;
+        if ((rc = launch_merge(A, fA, L, C, fC, L, L, perm, tie, B, fB, N, st))) return rc;
         if ((rc = launch_recount(p, B, b->counts.p + b->cnt_off[q], L, st))) return rc;
       } else {
-        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, A, fA, N, st))) return rc;
+        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, tie, A, fA, N, st))) return rc;
       }
     }
   }
@@ -1224,6 +1279,8 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (e == hipSuccess) e = b->clk.alloc((size_t)1 + n_probs);
   if (e == hipSuccess) e = b->margs.alloc((size_t)2 * n_probs);
   if (e == hipSuccess) e = b->items.alloc(items.size());
+  if (e == hipSuccess) e = b->tie.alloc((size_t)n_probs);
+  if (e == hipSuccess) e = hipMemset(b->tie.p, 0, sizeof(uint32_t) * n_probs);
   if (e != hipSuccess) {
     delete b;
     return fail(HEA_ENOMEM, std::string("batch allocation: ") + hipGetErrorString(e));

@@ -10,6 +10,19 @@ patterns) in golden_*.npz.  Only data is kept; no reference source.
 Cases cover: FCC (both shell sets) and BCC, K = 2..6, S = 1..3, 1x1x1 and
 2x2x2 cells (self neighbours and repeated neighbours), task in {0, 1, 4, 8},
 depth 1..4, swap chains with adjacent sites, and the full global loop.
+
+golden_c0.npz: the reference's own example configuration (examples/
+config.yaml: FCC 10x10x10 = 4000 sites, 4 x 1000 atoms, parallel_task 256,
+converge_depth 10) at full size, in both FCC shell sets (12/6/24 and the
+reference's 12/6/12): local search I of four permuted lattices and a short
+global loop (4 solutions, 2 iterations).  The problem itself is regenerated
+by pyhea_amd.problem.workload(0); a digest of its neighbour table and
+targets is stored to show it is the same one.
+
+golden_ties.npz: global loops whose populations hold equal fitness values
+with more than 16 lattices in a sort (2L > 16), where the reference's
+std::sort (accelerate.cpp:47) is not stable: the order it leaves is part of
+the result.
 """
 from __future__ import annotations
 
@@ -73,9 +86,82 @@ def adjacent_swaps(o: Oracle, lat, n, rng):
     return np.array(sw, np.int32)
 
 
+def problem_digest(spec) -> np.ndarray:
+    """sha256 of the neighbour table, counts, weights and target bits."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for a in (np.ascontiguousarray(spec.nbr, np.int32), np.array(spec.counts, np.int32),
+              np.array(spec.weight, np.float64), np.ascontiguousarray(spec.target, np.float64)):
+        h.update(a.tobytes())
+    return np.frombuffer(h.digest(), np.uint8).copy()
+
+
+C0_SHELLS = ("true", "pyhea")
+
+
+def make_c0():
+    """Full-size local search and a short global loop at the reference's
+    example configuration (workload 0), both FCC shell sets."""
+    from pyhea_amd.problem import workload
+
+    arrays = {}
+    for sh in C0_SHELLS:
+        spec, run = workload(0, shells=sh)
+        o = Oracle.from_spec(spec)
+        r = RefHarness(o)
+        T, D, seed = run["parallel_task"], run["converge_depth"], run["seed"]
+        lat_in = np.stack([o.permute(seed, 0, 1, l) for l in range(4)])
+        fit_in = r.fitness(lat_in)
+        lo, fo, words, _ = r.local_mc(lat_in, fit_in, T, D, seed, 0, 2)
+        arrays[f"{sh}/digest"] = problem_digest(spec)
+        arrays[f"{sh}/lat_in"] = lat_in
+        arrays[f"{sh}/fit_in_bits"] = bits(fit_in)
+        arrays[f"{sh}/ls1_lat"] = lo
+        arrays[f"{sh}/ls1_fit_bits"] = bits(fo)
+        arrays[f"{sh}/ls1_words"] = words
+        snaps, words, _ = r.run_hcs(4, 2, T, D, seed)
+        arrays[f"{sh}/hcs_lat"] = snaps[-1][0]
+        arrays[f"{sh}/hcs_fit_bits"] = np.stack([bits(sn[1]) for sn in snaps])
+        arrays[f"{sh}/hcs_words"] = words
+    np.savez_compressed(os.path.join(HERE, "golden_c0.npz"), **arrays)
+    print("wrote golden_c0.npz:", len(arrays), "arrays")
+
+
+TIE_PROBLEMS = [
+    # name, type, cell, fractions, shells, solutions
+    ("fcc2_k2_l12", "FCC", 2, [0.5, 0.5], "true", 12),
+    ("fcc1_k2_l12", "FCC", 1, [0.5, 0.5], "pyhea", 12),
+    ("bcc2_k2_l20", "BCC", 2, [0.5, 0.5], "true", 20),
+    ("fcc2_k2_l16", "FCC", 2, [0.75, 0.25], "true", 16),
+]
+
+
+def make_ties():
+    """Global loops with equal fitness keys among more than 16 lattices."""
+    arrays, index = {}, []
+    for pi, (name, typ, cell, fr, shells, L) in enumerate(TIE_PROBLEMS):
+        spec, arrs = problem_arrays(name, typ, cell, fr, shells, 3, seed=300 + pi)
+        arrays.update(arrs)
+        o = Oracle.from_spec(spec)
+        I, T, D, seed = 2, 4, 2, 60 + pi
+        snaps, words, _ = RefHarness(o).run_hcs(L, I, T, D, seed)
+        fits = np.stack([sn[1] for sn in snaps])
+        assert any(len(set(f.tolist())) < len(f) for f in fits), name  # keys do tie
+        arrays[f"{name}/hcs_lat"] = np.stack([sn[0] for sn in snaps])
+        arrays[f"{name}/hcs_fit_bits"] = bits(fits)
+        arrays[f"{name}/hcs_words"] = words
+        index.append(dict(problem=name, solutions=L, total_iter=I, task=T, depth=D, seed=seed))
+    arrays["index_json"] = np.frombuffer(json.dumps(index).encode(), np.uint8).copy()
+    np.savez_compressed(os.path.join(HERE, "golden_ties.npz"), **arrays)
+    print("wrote golden_ties.npz:", len(arrays), "arrays")
+
+
 def main():
     if not RefHarness.available():
         raise SystemExit("oracle/_ref/ref_hcs missing: run `make -C oracle ref` first")
+    make_c0()
+    make_ties()
     arrays, index = {}, {"problems": [], "local_mc": [], "run_hcs": []}
     for pi, (name, typ, cell, fr, shells, S) in enumerate(PROBLEMS):
         spec, arrs = problem_arrays(name, typ, cell, fr, shells, S, seed=100 + pi)
