@@ -116,6 +116,25 @@ int hea_local_mc_device(hea_problem p, int32_t solutions, void* d_lattices, void
                         int32_t parallel_task, int32_t converge_depth, uint64_t seed,
                         uint32_t iter, uint32_t phase, void* stream);
 
+/* Bytes of the fixed per-device workspace of hea_local_mc_device_noalloc. */
+#define HEA_NOALLOC_SCRATCH_BYTES (4u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as hea_local_mc_device,
+ * but the search's workspace (iteration word, per-lattice counters, pair
+ * counts) lives in a fixed array of HEA_NOALLOC_SCRATCH_BYTES that each
+ * device holds for the life of the process, and the call allocates nothing
+ * (no hipMalloc, hipFree or allocating library routine).  It needs
+ * 256 + roundup(32 * solutions, 256) + 4 * solutions * S * K * K bytes;
+ * larger calls return HEA_EINVAL.  The workspace keeps the last call's
+ * contents and is shared by every call on the device: calls that overlap on
+ * one device (other streams, other threads, other problems) race and must
+ * be serialised by the caller. */
+int hea_local_mc_device_noalloc(hea_problem p, int32_t solutions, void* d_lattices,
+                                void* d_fitness, int32_t parallel_task, int32_t converge_depth,
+                                uint64_t seed, uint32_t iter, uint32_t phase, void* stream);
+
 /* Fitness by full recount: accelerate::cpu::calculate_fitness_of_solutions
  * (accelerate.cpp:118-180). */
 int hea_fitness(hea_problem p, int32_t solutions, const uint8_t* lattices, double* out_fitness);

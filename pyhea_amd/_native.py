@@ -26,6 +26,7 @@ EXPORTS = (
     "hea_hcs_run",
     "hea_local_mc",
     "hea_local_mc_device",
+    "hea_local_mc_device_noalloc",
     "hea_fitness",
     "hea_pair_counts",
     "hea_generate",
@@ -104,6 +105,7 @@ def _declare(lib: C.CDLL) -> None:
             [P, I32, u8p, dp, I32, I32, U64, U32, U32, I32, C.POINTER(HeaStats)],
         ),
         "hea_local_mc_device": (C.c_int, [P, I32, P, P, I32, I32, U64, U32, U32, P]),
+        "hea_local_mc_device_noalloc": (C.c_int, [P, I32, P, P, I32, I32, U64, U32, U32, P]),
         "hea_fitness": (C.c_int, [P, I32, u8p, dp]),
         "hea_pair_counts": (C.c_int, [P, I32, u8p, i32p]),
         "hea_generate": (C.c_int, [P, I32, U64, U32, U32, u8p, dp]),

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <utility>
 #include <functional>
 #include <string>
@@ -21,6 +22,14 @@
 #include "../../include/pyhea_hcs.h"
 #include "hea_internal.h"
 #include "hea_sort.h"
+
+// Workspace of hea_local_mc_device_noalloc: the local search's iteration
+// word, per-lattice counters and pair counts live here instead of in
+// buffers the library allocates.  Each device holds one copy for the life of
+// the process; every call on a device shares it.  Host code never passes
+// `scratch` itself to a kernel (on the host it names a stand-in, not device
+// memory): it uses the address hipGetSymbolAddress returned for the device.
+static __device__ __align__(256) unsigned char scratch[HEA_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -37,6 +46,30 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                              \
       return fail(HEA_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
   } while (0)
+
+// Device address of `scratch`, one per device, resolved at problem creation
+// with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(HEA_EUNSUPPORTED, "device index >= 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return HEA_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(HEA_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+  return HEA_OK;
+}
+
+unsigned char* scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 // ------------------------------------------------------------------ kernels --
 
@@ -677,6 +710,7 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
   do {
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
+    if ((rc = resolve_scratch(device)) != HEA_OK) break;
     int lds_max = 0;
     e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
@@ -754,19 +788,30 @@ int hea_generate(hea_problem p, int32_t L, uint64_t seed, uint32_t iter, uint32_
   return HEA_OK;
 }
 
+// Device work of one local search (no sort): iteration word, zeroed
+// counters, pair-count recount, search.  iter_w [1], stats [4L] and counts
+// [L][S][K][K] are the caller's workspace; nothing is allocated here.
+static int local_mc_enqueue(hea_problem p, int L, uint8_t* lat, double* fit, int task, int depth,
+                            uint64_t seed, uint32_t iter, uint32_t phase, uint32_t* iter_w,
+                            int64_t* stats, int32_t* counts, hipStream_t st) {
+  size_t lds;
+  bool sp_lds;
+  int rc = mc_lds(p, task, &lds, &sp_lds);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, iter_w, iter);
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st, reinterpret_cast<int32_t*>(stats),
+                     (size_t)8 * L);
+  if ((rc = launch_recount(p, lat, counts, L, st))) return rc;
+  return launch_mc(p, lat, fit, counts, stats, iter_w, nullptr, L, task, depth, seed, phase, -1.0,
+                   lds, sp_lds, st);
+}
+
 static int local_mc_dev(hea_problem p, int L, uint8_t* lat, double* fit, int task, int depth,
                         uint64_t seed, uint32_t iter, uint32_t phase, int sort, hipStream_t st) {
   int rc = ensure_scratch(p, L);
   if (rc) return rc;
-  size_t lds;
-  bool sp_lds;
-  if ((rc = mc_lds(p, task, &lds, &sp_lds))) return rc;
-  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, p->d_iter.p, iter);
-  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st,
-                     reinterpret_cast<int32_t*>(p->d_stats.p), (size_t)8 * L);
-  if ((rc = launch_recount(p, lat, p->d_counts.p, L, st))) return rc;
-  if ((rc = launch_mc(p, lat, fit, p->d_counts.p, p->d_stats.p, p->d_iter.p, nullptr, L, task,
-                      depth, seed, phase, -1.0, lds, sp_lds, st)))
+  if ((rc = local_mc_enqueue(p, L, lat, fit, task, depth, seed, iter, phase, p->d_iter.p,
+                             p->d_stats.p, p->d_counts.p, st)))
     return rc;
   if (sort) {
     if ((rc = launch_merge(lat, fit, L, lat, fit, 0, L, p->d_perm.p, p->d_tie.p, p->d_lat2.p,
@@ -815,6 +860,31 @@ int hea_local_mc_device(hea_problem p, int32_t L, void* d_lattices, void* d_fitn
   HEA_HIP(hipSetDevice(p->device));
   return local_mc_dev(p, L, static_cast<uint8_t*>(d_lattices), static_cast<double*>(d_fitness),
                       task, depth, seed, iter, phase, 0, static_cast<hipStream_t>(stream));
+}
+
+int hea_local_mc_device_noalloc(hea_problem p, int32_t L, void* d_lattices, void* d_fitness,
+                                int32_t task, int32_t depth, uint64_t seed, uint32_t iter,
+                                uint32_t phase, void* stream) {
+  int rc = check_run_args(p, L, task, depth);
+  if (rc) return rc;
+  if (!d_lattices || !d_fitness) return fail(HEA_EINVAL, "null population");
+  // workspace: [iteration word, padded to 256 B][stats int64 4L, padded to
+  // 256 B][pair counts int32 L*S*K*K]
+  const size_t stats_b = ((size_t)32 * L + 255) & ~(size_t)255;
+  const size_t counts_b = (size_t)4 * L * p->hc.S * p->hc.K * p->hc.K;
+  const size_t need = 256 + stats_b + counts_b;
+  if (need > HEA_NOALLOC_SCRATCH_BYTES)
+    return fail(HEA_EINVAL, "solutions too large for the fixed workspace: needs " +
+                                std::to_string(need) + " bytes, HEA_NOALLOC_SCRATCH_BYTES = " +
+                                std::to_string((size_t)HEA_NOALLOC_SCRATCH_BYTES));
+  HEA_HIP(hipSetDevice(p->device));
+  unsigned char* ws = scratch_of(p->device);
+  if (!ws) return fail(HEA_EHIP, "workspace address of this device was not resolved");
+  return local_mc_enqueue(p, L, static_cast<uint8_t*>(d_lattices),
+                          static_cast<double*>(d_fitness), task, depth, seed, iter, phase,
+                          reinterpret_cast<uint32_t*>(ws), reinterpret_cast<int64_t*>(ws + 256),
+                          reinterpret_cast<int32_t*>(ws + 256 + stats_b),
+                          static_cast<hipStream_t>(stream));
 }
 
 // ------------------------------------------------------------------ session --
