@@ -1,23 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the SRO lattice search on one MI355X.
 
-Workload (default ``plain-bcc16``): the plain drop-in search on a BCC 16^3
-lattice (8192 sites), 5 species at fractions [0.30, 0.25, 0.20, 0.15, 0.10],
-3 shells (8/6/12), weights [4, 1, 0.5], seeded targets, solutions = 512,
-parallel_task = 256, converge_depth = 10.  One step = one global iteration
-of run_local_parallel_hcs (pyhea/cpp/accelerate.cpp:741-761): generate,
-local search I, merge, local search II, replayed as one HIP graph with the
-population resident in HBM.  --warmup 1 --steps 49 covers total_iter = 50.
+Default workload ``ttf-fcc30``, the headline comparison: FCC 30^3 = 108000
+sites, 5 equimolar species, shells 12/6/24, weights [4, 1, 0.5], seeded
+targets, solutions = 256, parallel_task = 512, converge_depth = 10.  Time to
+fitness <= 1e-3 of the gradient-guided search in the batched persistent
+launch (replayed as a HIP graph) against the plain GPU search and against the
+reference CPU engine.  One step = one global iteration of
+run_local_parallel_hcs (pyhea/cpp/accelerate.cpp:741-761): generate, local
+search I, merge, local search II.  The timed region starts a fresh population
+(so the device clock that stamps the first fitness <= 1e-3 starts inside it)
+and runs --steps iterations of the guided + batched search; `value` is its
+rate of swap evals (proposals walked and screened: rounds x parallel_task).
+The plain search then runs from its own fresh population until it reaches
+the threshold (or --budget-s), and the reference engine is timed on a bounded
+sample of the same workload on the host cores.
 
-Prints one JSON line: swap evals/s (value), the roofline of the dominant
-kernel (the local-search kernel, timed live with HIP events on its stream)
-and a CPU baseline: the reference engine (oracle/_ref/ref_hcs) timed on a
-bounded sample of the same workload on the host cores.
+Prints one JSON line: value, time to fitness of both modes and their ratios,
+the roofline of the dominant kernel (the guided batched local-search kernel,
+each launch timed with HIP events on its stream) and the CPU baseline.
 
-Other workloads (--workload): plain-fcc10; ttf-fcc30 / ttf-fcc50 (time to
-fitness, plain vs guided vs guided-batched); batched-64 (64 uneven problems
-in one persistent launch per phase); relax-fcc30 / relax-fcc50 (forward and
-backward of the differentiable relaxation).
+Other workloads (--workload): ttf-fcc50 (the same comparison at FCC 50^3,
+500000 sites, 6 species, 64 solutions); plain-bcc16 (plain drop-in search at
+BCC 16^3, 512 solutions, 50 iterations); plain-fcc10; batched-64 (64 uneven
+problems in one persistent launch per phase); relax-fcc30 / relax-fcc50
+(forward and backward of the differentiable relaxation).
 """
 from __future__ import annotations
 
@@ -62,9 +69,10 @@ def profiled_traffic(workload: str, kernel: str = "k_local_mc", key: str = "traf
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=49)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed global iterations (default: 5 for ttf-*, 49 for plain-*)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain-bcc16",
+    ap.add_argument("--workload", default="ttf-fcc30",
                     choices=["plain-bcc16", "plain-fcc10", "ttf-fcc30", "ttf-fcc50", "batched-64",
                              "relax-fcc30", "relax-fcc50"])
     ap.add_argument("--budget-s", type=float, default=60.0,
@@ -79,7 +87,7 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=3,
                     help="eager iterations timed per launch for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
-                    help="approximate CPU work of the baseline sample (0 disables it)")
+                    help="search seconds of each CPU-baseline process (0 disables it)")
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1),
                     help="CPU baseline processes (default: every host CPU, at most 16)")
     ap.add_argument("--no-tune", action="store_true",
@@ -109,61 +117,71 @@ def _host_cpu() -> str:
     return f"{model}, {os.cpu_count()} logical CPUs"
 
 
-def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0=None, fit0=None):
-    """Reference engine (or the oracle when the reference binary is absent)
-    running local search I on lattices of the same workload, one process per
-    core.  Eval counts come from the GPU run of the identical work (the two
-    are bit-identical, so the counts are the reference's)."""
+def cpu_baseline(wid: int, spec, run, target_secs: float, procs: int, lat0, fit0,
+                 start: str) -> dict:
+    """The reference engine (oracle/_ref/ref_hcs; the oracle's restatement when
+    the binary is absent) running local search I on the lattices lat0, one
+    process per core, each process on its own chunk of lattices.  A process
+    repeats its chunk with fresh proposal streams (seed + 1000 + r) until
+    about target_secs of its own search time have passed; the search time is
+    the engine's own clock around the reference's local search (process start
+    and file I/O excluded).  Swap evals come from the GPU search of exactly the
+    same chunks and streams (bit-identical decisions, so the counts are the
+    reference's).  value = sum over processes of evals / search seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle, RefHarness
+    from oracle import RefHarness
 
     from pyhea_amd.hcs import Problem
 
-    o = Oracle.from_spec(spec)
     kind = "reference" if RefHarness.available() else "port"
-    start = "random"
-    if lat0 is None:
-        # per-lattice cost of local search I on the reference engine, measured
-        # on an MI355X host: ~0.28 s of one core for BCC 16^3, T = 256
-        per_lat = 0.28
-        n_lat = max(procs, int(round(target_secs / per_lat)))
-        n_lat = (n_lat + procs - 1) // procs * procs
-        lat0 = np.stack([o.permute(run["seed"], 0, 1, l) for l in range(n_lat)])
-        fit0 = o.fitness(lat0)
-    else:
-        n_lat, start = len(lat0), "converged"
     T, D = run["parallel_task"], run["converge_depth"]
-    chunks = np.array_split(np.arange(n_lat), procs)
+    chunks = [c for c in np.array_split(np.arange(len(lat0)), procs) if len(c)]
     with tempfile.TemporaryDirectory() as tmp:
-        jobs = []
+        jobs, outs = [], []
         t0 = time.perf_counter()
         for ci, idx in enumerate(chunks):
-            f = os.path.join(tmp, f"c{ci}.npz")
+            f, fo = os.path.join(tmp, f"c{ci}.npz"), os.path.join(tmp, f"o{ci}.npy")
             np.savez(f, lat=lat0[idx], fit=fit0[idx])
+            outs.append(fo)
             jobs.append(subprocess.Popen([sys.executable, os.path.abspath(__file__),
-                                          "--cpu-worker", f, kind, str(T), str(D),
-                                          str(run["seed"]), str(wid)]))
+                                          "--cpu-worker", f, fo, kind, str(T), str(D),
+                                          str(run["seed"]), str(wid), str(target_secs)]))
         for j in jobs:
             if j.wait() != 0:
                 raise RuntimeError("cpu baseline process failed")
         wall = time.perf_counter() - t0
+        secs = [np.load(fo) for fo in outs]  # per process: search seconds of each repetition
     p = Problem.from_spec(spec)
-    _, _, st = p.local_mc(lat0, fit0, T, D, run["seed"], 0, 2, sort=False)
+    rate, evals, reps = 0.0, 0, 0
+    for idx, sc in zip(chunks, secs):
+        ev = 0
+        for r in range(len(sc)):
+            _, _, st = p.local_mc(lat0[idx], fit0[idx], T, D, run["seed"] + 1000 + r, 0, 2,
+                                  sort=False)
+            ev += st["evals"]
+        rate += ev / float(sc.sum())
+        evals += ev
+        reps += len(sc)
+    p.close()
+    search_s = float(sum(sc.sum() for sc in secs))
     return {
-        "value": st["evals"] / wall,
+        "value": rate,
         "unit": "swap evals/s",
-        "cores": procs,
+        "cores": len(chunks),
         "host": _host_cpu(),
         "kind": kind,
-        "sample": f"local search I (T={T}, depth={D}) on {n_lat} {start} "
-                  f"{spec.latt_typ} {spec.cell_dim}^3 lattices, "
-                  f"{procs} processes, {st['evals']} evals in {wall:.1f} s wall "
-                  f"(about {wall * procs:.0f} core-seconds)",
+        "sample": f"local search I (T={T}, depth={D}) of {len(lat0)} {start} "
+                  f"{spec.latt_typ} {spec.cell_dim}^3 lattices in {len(chunks)} processes "
+                  f"(one chunk each, {reps} chunk searches with fresh streams): {evals} evals "
+                  f"in {search_s:.1f} core-seconds of the engine's own search time, "
+                  f"{wall:.1f} s wall",
     }
 
 
-def cpu_worker(path: str, kind: str, T: int, D: int, seed: int, wid: int) -> None:
-    """One CPU-baseline process: local search I on its share of lattices."""
+def cpu_worker(path: str, out: str, kind: str, T: int, D: int, seed: int, wid: int,
+               target_secs: float) -> None:
+    """One CPU-baseline process: local search I on its chunk of lattices,
+    repeated with fresh streams until target_secs of search time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, RefHarness
 
@@ -172,84 +190,149 @@ def cpu_worker(path: str, kind: str, T: int, D: int, seed: int, wid: int) -> Non
     spec, _ = workload(wid)
     o = Oracle.from_spec(spec)
     d = np.load(path)
-    if kind == "reference":
-        RefHarness(o).local_mc(d["lat"], d["fit"], T, D, seed, 0, 2)
-    else:
-        o.local_mc(d["lat"], d["fit"], T, D, seed, 0, 2, sort=False)
+    secs = []
+    while not secs or (sum(secs) < target_secs and len(secs) < 1000):
+        r = len(secs)
+        if kind == "reference":
+            secs.append(RefHarness(o).local_mc(d["lat"], d["fit"], T, D, seed + 1000 + r, 0, 2)[3])
+        else:
+            t0 = time.perf_counter()
+            o.local_mc(d["lat"], d["fit"], T, D, seed + 1000 + r, 0, 2, sort=False)
+            secs.append(time.perf_counter() - t0)
+    np.save(out, np.array(secs, dtype=np.float64))
+
+
+def calib_copy(mb: int) -> None:
+    """Copies mb MiB device to device (a kernel of known size for calibrating
+    the HBM byte counters); nothing when mb <= 0."""
+    if mb <= 0:
+        return
+    import torch
+
+    src = torch.ones(mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def guided_launch_bytes(st: dict, T: int, sum_z: int, cand: int) -> int:
+    """Algorithmic bytes of a guided local-search launch: every round scores
+    T x cand candidate pairs (two 8-byte neighbour-count records and two
+    species bytes each), walks its T chosen proposals (bytes_per_eval each;
+    re-screens after an accept reuse the walks and read nothing), and every
+    accepted swap rewrites the records of both sites' neighbours (8 bytes
+    read + 8 written each)."""
+    return (st["rounds"] * T * (bytes_per_eval(sum_z) + cand * (2 * 8 + 2))
+            + st["accepts"] * 2 * sum_z * 16)
 
 
 def time_to_fitness(a):
-    """Plain vs guided search on one problem: global iterations until the
-    population best reaches --thr or --budget-s runs out; the device clock
-    stamps the first local-search accept at or below the threshold.  Guided
-    search runs twice: through a session (one local-search launch per phase)
-    and through the batched persistent launch replayed as a HIP graph
-    ("guided-batched"); both make the same decisions."""
+    """Headline: guided + batched vs plain time to fitness <= thr (see the
+    module docstring)."""
+    import torch
+
+    from pyhea_amd import _native
     from pyhea_amd.hcs import Batch, Problem
     from pyhea_amd.problem import workload
 
+    if _native.device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
     wid = {"ttf-fcc30": 3, "ttf-fcc50": 4}[a.workload]
     spec, run = workload(wid)
     L, T, D, seed = run["solutions"], run["parallel_task"], run["converge_depth"], run["seed"]
+    cand, acc = a.guide_candidates, a.guide_accepts
     prob = Problem.from_spec(spec)
-    modes = {}
-    for mode in ("plain", "guided", "guided-batched"):
-        if mode == "guided-batched":
-            s = Batch([prob], L, T, D, thr=a.thr, seed=seed)
-        else:
-            s = prob.session(L, T, D, thr=a.thr, seed=seed)
-        if mode != "plain":
-            s.guide(candidates=a.guide_candidates, tau=0.1, accepts=a.guide_accepts)
-        t0 = time.perf_counter()
-        s.init()
-        trace = []
-        while True:
-            s.step(1)
-            best = s.best()
-            trace.append((round(time.perf_counter() - t0, 3), best))
-            if best <= a.thr or time.perf_counter() - t0 > a.budget_s:
-                break
-        lat_f, fit, st = s.read() if mode != "guided-batched" else s.read(0)
-        if mode == "plain":
-            plain_final = (lat_f, fit)
-        # algorithmic bytes of the walked proposals (rounds x parallel_task;
-        # guided re-screens from stored walks read no neighbour rows)
-        walked_gbs = st["rounds"] * T * bytes_per_eval(spec.sum_z) / trace[-1][0] / 1e9
-        modes[mode] = {"best": float(fit.min()), "iterations": len(trace), "wall_s": trace[-1][0],
-                       "evals": st["evals"], "evals_per_s": st["evals"] / trace[-1][0],
-                       "achieved_gbs": round(walked_gbs, 1),
-                       "hbm_frac": round(walked_gbs / HBM_PEAK_GBS, 4),
-                       "time_to_thr_s": st["first_hit_s"] if st["first_hit_s"] >= 0 else None,
-                       "trace": trace}
-        s.close()
-    # time each mode needed to reach the plain run's final best
-    target = modes["plain"]["best"]
-    for m in modes.values():
-        m["time_to_plain_best_s"] = next((t for t, f in m["trace"] if f <= target), None)
-    tp = modes["plain"]["time_to_plain_best_s"]
-    out = {"metric": "time to fitness (guided vs plain)", "workload": a.workload,
-           "n_sites": spec.n_sites, "solutions": L, "parallel_task": T, "thr": a.thr,
-           "guide": {"candidates": a.guide_candidates, "tau": 0.1, "accepts": a.guide_accepts},
-           "modes": modes}
-    hp = modes["plain"]["time_to_thr_s"]
-    for m in ("guided", "guided-batched"):
-        tg, hg = modes[m]["time_to_plain_best_s"], modes[m]["time_to_thr_s"]
-        out[f"speedup_to_plain_best_{m}"] = (tp / tg) if (tp and tg) else None
-        out[f"speedup_to_thr_{m}"] = (hp / hg) if (hp and hg) else None
-    hg = modes["guided-batched"]["time_to_thr_s"]
-    if a.cpu_seconds > 0 and hp and hg:
-        # the reference engine makes plain mode's decisions, so it needs the
-        # evals plain mode needed; its rate is timed on lattices of the plain
-        # run's final population (bounded: they are converged)
-        procs = a.cpu_procs
-        n0 = min(L, 10 * procs)  # ~10-30 s of CPU work on converged lattices
-        cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, procs,
-                           lat0=plain_final[0][:n0], fit0=plain_final[1][:n0])
-        pm = modes["plain"]
-        evals_to_thr = pm["evals"] * hp / pm["wall_s"]  # plain's rate is steady
-        cpu["time_to_thr_projected_s"] = evals_to_thr / cpu["value"]
-        out["cpu_baseline"] = cpu
-        out["speedup_guided_batched_vs_cpu"] = cpu["time_to_thr_projected_s"] / hg
+
+    # guided + batched: persistent launch per phase, iteration replayed as a HIP graph
+    b = Batch([prob], L, T, D, thr=a.thr, seed=seed)
+    b.guide(candidates=cand, tau=0.1, accepts=acc)
+    b.init()
+    b.step(a.warmup)  # captures the graph; untimed
+    b.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.init()  # fresh population; starts the device clock
+    b.step(a.steps)
+    b.sync()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    _, gfit, gst = b.read(0)
+    walked = gst["rounds"] * T
+    # every persistent launch of the same iterations, eagerly, with HIP events
+    b.init()
+    ms, cnt = b.step_timed(a.steps)
+    b.close()
+    launches = [c for it in cnt for c in it]
+    byts = sum(guided_launch_bytes(c, T, spec.sum_z, cand) for c in launches)
+    ach = byts / (float(ms.sum()) * 1e-3) / 1e9
+
+    # plain GPU search from its own fresh population until thr (or the budget)
+    s = prob.session(L, T, D, thr=a.thr, seed=seed)
+    t2 = time.perf_counter()
+    s.init()
+    it = 0
+    while True:
+        s.step(1)
+        it += 1
+        best = s.best()
+        if best <= a.thr or time.perf_counter() - t2 > a.budget_s:
+            break
+    pwall = time.perf_counter() - t2
+    plat, pfit, pst = s.read()
+    s.close()
+    ttf_g = gst["first_hit_s"] if gst["first_hit_s"] >= 0 else None
+    ttf_p = pst["first_hit_s"] if pst["first_hit_s"] >= 0 else None
+    plain = {"time_to_thr_s": ttf_p, "iterations": it, "wall_s": round(pwall, 3),
+             "best": float(pfit.min()), "swap_evals": pst["evals"],
+             "swap_evals_per_s": pst["evals"] / pwall}
+    guided = {"time_to_thr_s": ttf_g, "iterations": a.steps, "best": float(gfit.min()),
+              "swap_evals": walked, "swap_evals_per_s": walked / dt,
+              "screens_incl_rescreens": gst["evals"], "accepts": gst["accepts"],
+              "rounds": gst["rounds"], "candidates": cand, "tau": 0.1, "accepts_per_round": acc}
+    cpu = None
+    if a.cpu_seconds > 0:
+        # the reference makes plain mode's decisions, so it needs the evals
+        # plain mode needed; its rate is measured on the plain run's final
+        # (converged) lattices, repeated with fresh streams
+        n0 = min(L, 16 * a.cpu_procs)
+        cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, a.cpu_procs, plat[:n0], pfit[:n0],
+                           "converged")
+        if ttf_p:
+            evals_to_thr = pst["evals"] / pwall * ttf_p  # plain's rate is steady
+            cpu["time_to_thr_projected_s"] = evals_to_thr / cpu["value"]
+            cpu["projection"] = ("plain mode's swap evals up to its first fitness <= thr "
+                                 "divided by the measured reference rate; not run to the end")
+    ttf = {"thr": a.thr, "guided_batched_s": ttf_g, "plain_s": ttf_p,
+           "speedup_vs_plain": (ttf_p / ttf_g) if (ttf_p and ttf_g) else None,
+           "cpu_projected_s": cpu.get("time_to_thr_projected_s") if cpu else None}
+    ttf["speedup_vs_cpu"] = (ttf["cpu_projected_s"] / ttf_g) if (ttf["cpu_projected_s"]
+                                                                  and ttf_g) else None
+    prob.close()
+    calib_copy(a.calib_copy_mb)
+    out = {
+        "metric": "swap evals/s; time to fitness<=1e-3: guided+batched vs plain MC, 1 MI355X",
+        "value": walked / dt, "unit": "swap evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": a.workload, "lattice": f"{spec.latt_typ} {spec.cell_dim}^3",
+                   "n_sites": spec.n_sites, "species": spec.counts, "shells": spec.shell_sizes,
+                   "solutions": L, "parallel_task": T, "converge_depth": D, "seed": seed,
+                   "mode": "guided + batched (persistent launch, HIP graph)"},
+        "time_to_fitness": ttf,
+        "guided_batched": guided,
+        "plain": plain,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(f"guided-{a.workload[4:]}", "k_local_mc_batch"),
+                     "kernel": "k_local_mc_batch (guided)",
+                     "launches": len(launches),
+                     "avg_launch_ms": round(float(ms.mean()), 3),
+                     "bytes_per_launch": int(byts // len(launches)),
+                     "byte_model": "rounds*T*(B_eval + cand*18) + accepts*2*sumZ*16",
+                     "bytes_per_eval": bytes_per_eval(spec.sum_z)},
+        "cpu_baseline": cpu,
+    }
     print(json.dumps(out))
 
 
@@ -434,10 +517,13 @@ def relax_cpu_baseline(spec, th, dn, target_secs: float, threads: int):
 
 
 def main():
-    if len(sys.argv) == 8 and sys.argv[1] == "--cpu-worker":
-        cpu_worker(sys.argv[2], sys.argv[3], *map(int, sys.argv[4:8]))
+    if len(sys.argv) == 10 and sys.argv[1] == "--cpu-worker":
+        cpu_worker(sys.argv[2], sys.argv[3], sys.argv[4], *map(int, sys.argv[5:9]),
+                   float(sys.argv[9]))
         return
     a = parse()
+    if a.steps is None:
+        a.steps = 5 if a.workload.startswith("ttf-") else 49
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the population would shard one sub-population per GPU)"}))
@@ -514,17 +600,13 @@ def main():
         "avg_launch_ms": round(float(ms.mean()), 4),
         "evals_per_launch": int(ev.sum() // n_launch),
     }
-    if a.calib_copy_mb > 0:
-        import torch
-
-        n = a.calib_copy_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+    calib_copy(a.calib_copy_mb)
     cpu = None
     if a.cpu_seconds > 0:
-        cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, a.cpu_procs)
+        # random lattices of the workload (the product's keyed permutations,
+        # stream phase "new population"), four per process
+        lat_r, fit_r = prob.generate(4 * a.cpu_procs, seed=seed, iter=0, phase=1)
+        cpu = cpu_baseline(wid, spec, run, a.cpu_seconds, a.cpu_procs, lat_r, fit_r, "random")
     out = {
         "metric": "swap evals/s",
         "value": value,

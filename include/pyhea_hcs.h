@@ -212,6 +212,13 @@ int hea_batch_tile(hea_batch b, int32_t tile);
 int hea_batch_guide(hea_batch b, int32_t candidates, double tau, int32_t accepts);
 int hea_batch_init(hea_batch b);
 int hea_batch_step(hea_batch b, int32_t n_iter);
+/* Measurement helper with no counterpart in the reference: runs n_iter
+ * global iterations eagerly (no graph) with HIP events around each
+ * persistent local-search launch.  out_ms[2i], out_ms[2i+1]: durations (ms)
+ * of the local search I and II launches of iteration i; out_stats[2i],
+ * out_stats[2i+1]: their counters summed over the batch's problems
+ * (first_hit_s = -1).  Synchronous. */
+int hea_batch_step_timed(hea_batch b, int32_t n_iter, float* out_ms, hea_stats* out_stats);
 int hea_batch_sync(hea_batch b);
 /* Population (sorted) and counters of problem p. */
 int hea_batch_read(hea_batch b, int32_t p, uint8_t* out_lattices, double* out_fitness,

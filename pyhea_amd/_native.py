@@ -46,6 +46,7 @@ EXPORTS = (
     "hea_batch_guide",
     "hea_batch_init",
     "hea_batch_step",
+    "hea_batch_step_timed",
     "hea_batch_sync",
     "hea_batch_read",
     "hea_batch_destroy",
@@ -126,6 +127,8 @@ def _declare(lib: C.CDLL) -> None:
         "hea_batch_guide": (C.c_int, [P, I32, D, I32]),
         "hea_batch_init": (C.c_int, [P]),
         "hea_batch_step": (C.c_int, [P, I32]),
+        "hea_batch_step_timed": (C.c_int, [P, I32, C.POINTER(C.c_float),
+                                           C.POINTER(HeaStats)]),
         "hea_batch_sync": (C.c_int, [P]),
         "hea_batch_read": (C.c_int, [P, I32, u8p, dp, C.POINTER(HeaStats)]),
         "hea_batch_destroy": (C.c_int, [P]),

@@ -1228,8 +1228,9 @@ struct hea_batch_s {
 
 extern "C" int hea_batch_grid(hea_batch b, int32_t grid);  // autotuner hook (below)
 
-static int batch_enqueue_iteration(hea_batch b) {
+static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
   const int L = b->L, P = (int)b->probs.size();
+  const size_t sn = (size_t)4 * L * P;  // stats entries of the whole batch
   hipStream_t st = b->stream;
   int rc;
   for (int q = 0; q < P; ++q) {
@@ -1250,8 +1251,20 @@ static int batch_enqueue_iteration(hea_batch b) {
   for (int phase = HEA_PHASE_LS1; phase <= HEA_PHASE_LS2; ++phase) {
     ba.probs = b->margs.p + (size_t)(phase - HEA_PHASE_LS1) * P;
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, b->queue.p, 0u);
+    const int h = phase - HEA_PHASE_LS1;
+    if (hk) {
+      if (h == 0)
+        HEA_HIP(hipMemcpyAsync(hk->snap, b->stats.p, sizeof(int64_t) * sn,
+                               hipMemcpyDeviceToDevice, st));
+      HEA_HIP(hipEventRecord(hk->ev[2 * h], st));
+    }
     HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->guide.hist != nullptr, b->grid,
                                       b->threads, b->lds, st));
+    if (hk) {
+      HEA_HIP(hipEventRecord(hk->ev[2 * h + 1], st));
+      HEA_HIP(hipMemcpyAsync(hk->snap + (h + 1) * sn, b->stats.p, sizeof(int64_t) * sn,
+                             hipMemcpyDeviceToDevice, st));
+    }
     for (int q = 0; q < P; ++q) {
       hea_problem p = b->probs[q];
       const int N = p->hc.N;
@@ -1522,6 +1535,43 @@ int hea_batch_step(hea_batch b, int32_t n_iter) {
   }
   for (int i = 0; i < n_iter; ++i) HEA_HIP(hipGraphLaunch(b->exec, b->stream));
   return HEA_OK;
+}
+
+int hea_batch_step_timed(hea_batch b, int32_t n_iter, float* out_ms, hea_stats* out_stats) {
+  if (!b || n_iter < 0 || !out_ms || !out_stats) return fail(HEA_EINVAL, "bad arguments");
+  HEA_HIP(hipSetDevice(b->probs[0]->device));
+  const size_t sn = (size_t)4 * b->L * b->probs.size();
+  IterHooks hk;
+  for (auto& e : hk.ev) HEA_HIP(hipEventCreate(&e));
+  DevBuf<int64_t> snap;
+  std::vector<int64_t> h(3 * sn);
+  int rc = HEA_OK;
+  hipError_t e = snap.alloc(3 * sn);
+  if (e != hipSuccess) rc = fail(HEA_ENOMEM, hipGetErrorString(e));
+  hk.snap = snap.p;
+  for (int i = 0; i < n_iter && rc == HEA_OK; ++i) {
+    if ((rc = batch_enqueue_iteration(b, &hk))) break;
+    e = hipStreamSynchronize(b->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&out_ms[2 * i], hk.ev[0], hk.ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&out_ms[2 * i + 1], hk.ev[2], hk.ev[3]);
+    if (e == hipSuccess)
+      e = hipMemcpy(h.data(), snap.p, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(HEA_EHIP, hipGetErrorString(e));
+      break;
+    }
+    for (int ph = 0; ph < 2; ++ph) {
+      const std::vector<int64_t> d0(h.begin() + ph * sn, h.begin() + (ph + 1) * sn);
+      const std::vector<int64_t> d1(h.begin() + (ph + 1) * sn, h.begin() + (ph + 2) * sn);
+      hea_stats a, c;
+      sum_stats(d0, (int)(sn / 4), &a);
+      sum_stats(d1, (int)(sn / 4), &c);
+      out_stats[2 * i + ph] = {c.evals - a.evals, c.rounds - a.rounds, c.accepts - a.accepts,
+                               c.draws - a.draws, -1.0};
+    }
+  }
+  for (auto& ev : hk.ev) (void)hipEventDestroy(ev);
+  return rc;
 }
 
 int hea_batch_sync(hea_batch b) {

@@ -214,6 +214,15 @@ class Batch:
     def step(self, n_iter: int = 1) -> None:
         N.check(N.lib().hea_batch_step(self._h, n_iter))
 
+    def step_timed(self, n_iter: int = 1):
+        """Eager iterations with the duration of every persistent launch:
+        (ms [n, 2], counters [n, 2] as dicts summed over the problems)."""
+        ms = np.zeros((n_iter, 2), dtype=np.float32)
+        st = (N.HeaStats * (2 * n_iter))()
+        N.check(N.lib().hea_batch_step_timed(self._h, n_iter,
+                                             ms.ctypes.data_as(C.POINTER(C.c_float)), st))
+        return ms, [[st[2 * i + h].as_dict() for h in range(2)] for i in range(n_iter)]
+
     def sync(self) -> None:
         N.check(N.lib().hea_batch_sync(self._h))
 

@@ -28,3 +28,9 @@ def test_profiled_traffic_is_committed_for_measured_workloads():
         t = bench.profiled_traffic(workload, kernel, key)
         assert isinstance(t, int) and t > 0, workload
     assert bench.profiled_traffic("plain-fcc10") is None
+
+
+def test_guided_launch_bytes():
+    # one round of 512 proposals with 8 candidates each and two accepts, FCC 12/6/24
+    st = {"rounds": 1, "accepts": 2}
+    assert bench.guided_launch_bytes(st, 512, 42, 8) == 512 * (422 + 8 * 18) + 2 * 84 * 16
