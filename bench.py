@@ -50,12 +50,13 @@ def bytes_per_eval(sum_z: int) -> int:
     return 2 * sum_z * (4 + 1) + 2
 
 
-def profiled_traffic(workload: str, kernel: str = "k_local_mc", key: str = "traffic_bytes_per_launch"):
+def profiled_traffic(workload: str, kernel: str = "k_local_mc", key: str = "traffic_bytes_per_launch",
+                     fname: str | None = None):
     """HBM bytes per launch (or per step) from the committed rocprofv3 byte
     counters (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known
     size), or None when no profile covers the workload."""
     name = workload.replace("-", "_").replace("relax_", "")
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{name}.md")
+    path = os.path.join(ROOT, "profiles", fname or f"{kernel}_{name}.md")
     try:
         with open(path) as f:
             for line in f:
@@ -324,7 +325,10 @@ def time_to_fitness(a):
         "plain": plain,
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(f"guided-{a.workload[4:]}", "k_local_mc_batch"),
+                     "traffic": (profiled_traffic("guided-fcc30", "k_local_mc_batch")
+                                 if a.workload == "ttf-fcc30" else
+                                 profiled_traffic("", key="guided_traffic_bytes_per_launch",
+                                                  fname="k_local_mc_fcc50.md")),
                      "kernel": "k_local_mc_batch (guided)",
                      "launches": len(launches),
                      "avg_launch_ms": round(float(ms.mean()), 3),
