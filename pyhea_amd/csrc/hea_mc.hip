@@ -234,7 +234,7 @@ template <int KMAX>
 struct LdsLayout {
   static constexpr int TAB = HEA_MAX_S * KMAX * KMAX;
   using PT = typename Pack<KMAX>::T;
-  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, gkey, cmap, WA, WB, WC, cl, sh, sp, total;
+  size_t G, E0, R, tgt, SC, H, conc, xs, ex, xw, prop, gkey, cmap, WA, WB, WC, cl, sl, sh, sp, total;
   __host__ __device__ LdsLayout(int task, int n, bool sp_lds) {
     size_t o = 0;
     auto take = [&o](size_t bytes) {
@@ -259,6 +259,7 @@ struct LdsLayout {
     WB = take(sizeof(PT) * HEA_MAX_S * (size_t)task);
     WC = take(sizeof(uint32_t) * HEA_MAX_S * (size_t)task);
     cl = take(sizeof(int) * (size_t)task);
+    sl = take(sizeof(int) * 64);
     sh = take(sizeof(McShared));
     sp = o;
     total = sp_lds ? ((o + (size_t)n + 15) & ~(size_t)15) : o;
@@ -287,6 +288,7 @@ struct Lds {
   PT* WB;        // [S][task] histogram of a2's neighbours
   uint32_t* WC;  // [S][task] walk_pack(c11, c12, c21, c22, a, b)
   int* cl;       // [task] exact-stage candidate list
+  int* sl;       // [64] guided mode: the round's shortlist for further accepts
   McShared* sh;
   uint8_t* sp;   // [N] when resident
   static size_t bytes(int task, int n, bool sp_lds) { return LdsLayout<KMAX>(task, n, sp_lds).total; }
@@ -309,6 +311,7 @@ struct Lds {
     WB = reinterpret_cast<PT*>(base + L.WB);
     WC = reinterpret_cast<uint32_t*>(base + L.WC);
     cl = reinterpret_cast<int*>(base + L.cl);
+    sl = reinterpret_cast<int*>(base + L.sl);
     sh = reinterpret_cast<McShared*>(base + L.sh);
     sp = sp_lds ? base + L.sp : nullptr;
   }
@@ -389,6 +392,9 @@ __device__ __forceinline__ double wave_min(double v) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Largest v over all 64 lanes (every lane active), returned to every lane.
+__device__ __forceinline__ double wave_max(double v) { return -wave_min(-v); }
+
 // Refreshes the tables of entry (s,i,j) from prev_gamma and records its
 // screen error bounds.
 //
@@ -413,8 +419,14 @@ __device__ __forceinline__ double wave_min(double v) {
 // beta = beta0 + 4 u E_up, any proposal with X > Xmin + 2 beta has
 // err > err(Pmin)(1 + 8u), hence sqrt(err) rounds strictly above the
 // minimum's F and cannot be the reference's choice.
+struct EntryBounds {
+  double P, ub;  // the entry's pair error bound and its bound on |d'|
+  float hz;      // guided mode: Z_s |H_ij|
+};
+
 template <int KMAX, bool GUIDED>
-__device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int i, int j) {
+__device__ __forceinline__ EntryBounds refresh_entry_core(const HeaMcArgs& A, const Lds<KMAX>& T,
+                                                          int s, int i, int j) {
   const int e = T.idx(s, i, j);
   const double g = T.G[e];
   const double dd = T.conc[i] * T.conc[j];
@@ -441,16 +453,24 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
                    2.0 * e1 * (D1 + 2.0 * Ae + e1) + 2.0 * kU * t1 * t1 +
                    2.0 * kU * (Ae + e2) * (D1 + Ae + e2);
   const double ub = D1 + Ae + e1 + e2;
-  atomicMax(&T.sh->bmax[s], (unsigned long long)__double_as_longlong(P * 1.0001));
-  atomicMax(&T.sh->ubmax[s], (unsigned long long)__double_as_longlong(ub * 1.0001));
+  EntryBounds r{P * 1.0001, ub * 1.0001, 0.0f};
   if constexpr (GUIDED) {
     // H_ij = dF2/dn_ij + dF2/dn_ji of the fitness squared
     // (accelerate.cpp:556-569) with gamma = n delta: dF2/dn_ij =
     // -2 w (alpha_ij - t_ij) delta / dd_ij = 2 w d_ij c_ij, c_ij = -delta/dd_ij
     const float h = (float)(2.0 * A.weight[s] * (dij + dji) * T.SC[e].y);
     T.H[T.hidx(s, i, j)] = h;
-    atomicMax(&T.sh->gmax, __float_as_uint(fabsf(h) * (float)A.Z[s]));
+    r.hz = fabsf(h) * (float)A.Z[s];
   }
+  return r;
+}
+
+template <int KMAX, bool GUIDED>
+__device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int i, int j) {
+  const EntryBounds r = refresh_entry_core<KMAX, GUIDED>(A, T, s, i, j);
+  atomicMax(&T.sh->bmax[s], (unsigned long long)__double_as_longlong(r.P));
+  atomicMax(&T.sh->ubmax[s], (unsigned long long)__double_as_longlong(r.ub));
+  if constexpr (GUIDED) atomicMax(&T.sh->gmax, __float_as_uint(r.hz));
 }
 
 // beta of the refreshed tables (see refresh_entry).
@@ -589,6 +609,398 @@ __device__ __forceinline__ double current_fitness(const HeaMcArgs& A, const Lds<
 __device__ __forceinline__ bool key_less(unsigned long long f1, int j1, unsigned long long f2,
                                          int j2) {
   return (f1 < f2) || (f1 == f2 && j1 < j2);
+}
+
+// Exact stage of a round, run by one wave (wave 0): the candidates T.cl[0,
+// nsup) (a superset of the proposals whose X lies within 2 beta of xmin),
+// first strict minimum of F (accelerate.cpp:663-674) and the accept test
+// (:678).  Writes the decision to T.sh (res_kind, res_j, res_a, res_b,
+// res_key, pf, pfm, fest) and, for an accept, the winner's new gamma rows;
+// with `reset` an accept also clears the bound maxima that the refresh of
+// every entry then takes again.
+template <int KMAX, bool EXACT>
+__device__ __forceinline__ void exact_stage(const HeaMcArgs& A, const Lds<KMAX>& T,
+                                            const uint8_t* __restrict__ sp, int lane, int task,
+                                            int nsup, double xmin, double beta, bool reset) {
+  const int K = A.K, S = A.S;
+  double prev_fit = T.sh->pf, fest = T.sh->fest;
+  int pfm = T.sh->pfm;
+  double thr = xmin + 2.0 * beta;
+  if (!(thr < __builtin_inf())) thr = __builtin_inf();
+  // filter the waves' lists against the workgroup's window, compacting
+  // in place (every entry is read before any of its chunk is written)
+  int nc = 0, jfirst = -1;  // jfirst = cl[0] after the compaction
+  for (int k0 = 0; k0 < nsup; k0 += 64) {
+    const int j = (k0 + lane < nsup) ? T.cl[k0 + lane] : -1;
+    const bool f = j >= 0 && !(T.xs[j] > thr);
+    const unsigned long long bal = __ballot(f);
+    if (nc == 0 && bal) jfirst = __builtin_amdgcn_readlane(j, (int)__builtin_ctzll(bal));
+    wave_sync();
+    if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+    nc += __popcll(bal);
+  }
+  wave_sync();
+  // lane (c, s, x): candidate c of the chunk, shell s, species x
+  const int GS = S * K;      // lanes per candidate
+  const int cpc = 64 / GS;   // candidates per chunk
+  const int g = lane / K, x = lane - g * K;
+  const int c = g / S, s = g - c * S;
+  constexpr int XW = 4 + KMAX;
+  double* slot = T.xw + lane * XW;
+  const double* grp = T.xw + g * K * XW;  // lane (c, s, y) at grp + y * XW
+  unsigned long long bk = ~0ull;
+  int bj = 0x7fffffff, bc = 0, wc0 = -1;
+  int ra = 0, rb = 0;  // window-decided accept: the winner's species (lanes < S*K)
+  // When prev_fit = F(gamma), the window decides most rounds: every
+  // proposal's err lies within beta of E + X (E common, X = 0 for the
+  // current lattice), and 2 beta keeps sqrt's rounding out of the way.
+  // One candidate with X < -2 beta is certainly accepted; X > 2 beta for
+  // the smallest X is certainly a reject.  Near the threshold, or
+  // otherwise, the exact path decides.
+  const double mg = 2.0 * beta;
+  const bool near = A.thr > 0.0 && !(fest + xmin > A.thr * A.thr * 1.01);
+  int kind = -1;
+  if (pfm != 0 && !near) {
+    if (nc == 1 && xmin + mg < 0.0) kind = 2;
+    else if (xmin - mg > 0.0) kind = 0;
+  }
+  if (kind == -1 && pfm == 2) {  // the exact path compares with prev_fit
+    prev_fit = current_fitness<KMAX, EXACT>(A, T, lane);
+    pfm = 1;
+  }
+  if (kind == 0) {
+    if (lane == 0) T.sh->res_kind = 0;
+    return;
+  }
+  for (int c0 = 0; c0 < nc; c0 += cpc) {
+    int j = -1;
+    if (nc == 1)  // the usual case: the candidate came with the filter's ballot
+      j = (lane < GS) ? jfirst : -1;
+    else if (lane < cpc * GS && c0 + c < nc)
+      j = T.cl[c0 + c];
+    int a = 0, b = 0;
+    double nA = 0.0, nB = 0.0;
+    if (j >= 0) {
+      const int2 pr = T.prop[j];  // only the rare replay path reads the sites
+      const int w = s * task + j;
+      const WalkRec r = walk_unpack(T.WC[w]);
+      a = r.a;
+      b = r.b;
+      exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w], r.c11, r.c12,
+                          r.c21, r.c22, nA, nB);
+      slot[0] = nA;
+      slot[1] = nB;
+      if (kind != 2) {  // alphas feed the terms, which a decided accept skips
+        slot[2] = 1.0 - nA / (T.conc[a] * T.conc[x]);
+        slot[3] = 1.0 - nB / (T.conc[b] * T.conc[x]);
+      }
+    }
+    wave_sync();
+    if (kind == 2) {  // the one candidate is accepted: its new rows only
+      if (j >= 0) {
+        T.G[T.idx(s, a, x)] = nA;
+        T.G[T.idx(s, x, a)] = nA;
+        T.G[T.idx(s, b, x)] = nB;
+        T.G[T.idx(s, x, b)] = nB;
+      }
+      bj = jfirst;
+      ra = a;
+      rb = b;
+      break;
+    }
+    if (j >= 0) {  // terms of row x (accelerate.cpp:560-566)
+      const double* tg = T.tgt + T.idx(s, x, 0);
+      const double* e0 = T.E0 + T.idx(s, x, 0);
+      const double sAx = slot[2], sBx = slot[3];
+#pragma unroll
+      for (int y = 0; y < KMAX; ++y) {
+        if (EXACT || y < K) {
+          const bool chg = (x == a) || (x == b) || (y == a) || (y == b);
+          const double sro = (x == a)   ? grp[y * XW + 2]
+                             : (x == b) ? grp[y * XW + 3]
+                             : (y == a) ? sAx
+                                        : sBx;
+          const double df = sro - tg[y];
+          slot[4 + y] = chg ? df * df : e0[y];
+        }
+      }
+    }
+    wave_sync();
+    if (j >= 0 && x == 0) {  // the shell's K*K terms in row-major order
+      double es = 0.0;
+#pragma unroll
+      for (int i = 0; i < KMAX; ++i)
+#pragma unroll
+        for (int y = 0; y < KMAX; ++y)
+          if (EXACT || (i < K && y < K)) es = es + grp[i * XW + 4 + y];
+      T.ex[g] = es;
+    }
+    wave_sync();
+    unsigned long long rk = ~0ull;
+    int rj = 0x7fffffff, rc = 0;
+    if (j >= 0 && s == 0 && x == 0) {
+      double err = 0.0;
+      for (int t = 0; t < S; ++t) err = err + A.weight[t] * T.ex[c * S + t];
+      rk = (unsigned long long)__double_as_longlong(sqrt(err));
+      rj = j;
+      rc = c;
+    }
+    if (nc - c0 == 1) {  // the usual case: one candidate, in lane 0
+      rk = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(rk >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)rk);
+      rj = __builtin_amdgcn_readfirstlane(rj);
+      rc = 0;
+    } else {
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        const unsigned long long ok =
+            ((unsigned long long)(uint32_t)__shfl_xor((int)(rk >> 32), m) << 32) |
+            (uint32_t)__shfl_xor((int)(uint32_t)rk, m);
+        const int oj = __shfl_xor(rj, m);
+        const int oc = __shfl_xor(rc, m);
+        if (key_less(ok, oj, rk, rj)) {
+          rk = ok;
+          rj = oj;
+          rc = oc;
+        }
+      }
+    }
+    if (key_less(rk, rj, bk, bj)) {
+      bk = rk;
+      bj = rj;
+      bc = rc;
+      wc0 = c0;
+    }
+    if (c0 + cpc < nc) wave_sync();  // the next chunk reuses xw and ex
+  }
+  if (kind == 2) {
+    if (reset && lane < HEA_MAX_S) {  // the refresh below takes new maxima
+      T.sh->bmax[lane] = 0ull;
+      T.sh->ubmax[lane] = 0ull;
+    }
+    if (lane == 0) {
+      if (reset) T.sh->gmax = 0u;
+      T.sh->res_kind = 2;
+      T.sh->res_j = bj;
+      T.sh->res_a = ra;
+      T.sh->res_b = rb;
+      T.sh->pfm = 2;
+      T.sh->fest = fest + xmin;
+    }
+    return;
+  }
+  const double best_fit = __longlong_as_double((long long)bk);
+  int a = 0, b = 0;
+  if (best_fit < prev_fit) {  // accept: write the winner's rows
+    const int2 pr = T.prop[bj];
+    a = sp[pr.x];
+    b = sp[pr.y];
+    if (wc0 + cpc < nc) {  // a later chunk overwrote the winner's entries
+      if (lane < GS) {
+        const int w = s * task + bj;
+        const WalkRec r = walk_unpack(T.WC[w]);
+        double nA, nB;
+        exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w], r.c11,
+                            r.c12, r.c21, r.c22, nA, nB);
+        slot[0] = nA;
+        slot[1] = nB;
+      }
+      bc = 0;
+    }
+    wave_sync();
+    // rows a and b of every shell, with their mirrors (columns a and b)
+    for (int e = lane; e < S * 2 * K; e += 64) {
+      const int t = e / (2 * K), h = (e / K) & 1, y = e % K;
+      const double v = T.xw[((bc * S + t) * K + y) * XW + h];
+      const int r = h ? b : a;
+      T.G[T.idx(t, r, y)] = v;
+      T.G[T.idx(t, y, r)] = v;
+    }
+    if (reset && lane < HEA_MAX_S) {  // the refresh below takes new maxima
+      T.sh->bmax[lane] = 0ull;
+      T.sh->ubmax[lane] = 0ull;
+    }
+    if (reset && lane == 0) T.sh->gmax = 0u;
+  }
+  if (lane == 0) {
+    const bool acc1 = best_fit < prev_fit;
+    T.sh->res_kind = acc1 ? 1 : 0;
+    T.sh->res_key = bk;
+    T.sh->res_j = bj;
+    T.sh->res_a = a;
+    T.sh->res_b = b;
+    T.sh->pf = acc1 ? best_fit : prev_fit;
+    T.sh->pfm = acc1 ? 1 : pfm;
+    if (acc1) T.sh->fest = best_fit * best_fit;
+  }
+}
+
+// Guided mode: the neighbours of a swap's two sites, work item t of
+// [0, 2 sum Z) (a1's lists, then a2's) in lane t mod stride, slot t / stride;
+// -1 where there is no item.  2 sum Z <= 186 <= 3 x 64, so three slots cover
+// every item for any stride >= 64.
+struct SwapNbrs {
+  int nb[3];
+};
+__device__ __forceinline__ SwapNbrs swap_nbrs_load(const HeaMcArgs& A, int2 pr, int first,
+                                                   int stride) {
+  int zs = 0;
+  for (int s = 0; s < A.S; ++s) zs += A.Z[s];
+  SwapNbrs r;
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int t = first + u * stride;
+    r.nb[u] = -1;
+    if (t < 2 * zs) {
+      const int site = t < zs ? pr.x : pr.y;
+      int q = t < zs ? t : t - zs, s = 0;
+      while (q >= A.Z[s]) q -= A.Z[s++];
+      r.nb[u] = A.nbr[(size_t)site * A.stride + A.off[s] + q];
+    }
+  }
+  return r;
+}
+
+// Guided mode, after swapping a1 = pr.x (species a -> b) and a2 = pr.y
+// (b -> a): every neighbour of the two sites (n, from swap_nbrs_load with the
+// same first and stride) sees the change in its neighbour-count record, and
+// the two sites and their neighbours are marked in the round's conflict map.
+template <int KMAX>
+__device__ __forceinline__ void guided_apply_swap(const HeaMcArgs& A, const Lds<KMAX>& T,
+                                                  unsigned long long* hist,
+                                                  const HeaHistLayout& hl, int2 pr, int a, int b,
+                                                  int first, int stride, const SwapNbrs& n) {
+  const int K = A.K;
+  int zs = 0;
+  for (int s = 0; s < A.S; ++s) zs += A.Z[s];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    if (n.nb[u] < 0) continue;
+    const int t = first + u * stride;
+    int q = t < zs ? t : t - zs, s = 0;
+    while (q >= A.Z[s]) q -= A.Z[s++];
+    // the last species has no field (its count is implied)
+    const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + hl.o[s]) : 0ull;
+    const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + hl.o[s]) : 0ull;
+    const unsigned long long dv = t < zs ? fb - fa : fa - fb;  // modulo 2^64
+    // every access to a lattice's records comes from its own workgroup:
+    // workgroup-scope atomics, loads and barriers order them
+    __hip_atomic_fetch_add(hist + n.nb[u], dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t mb = map_bit(n.nb[u]);
+    atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
+  }
+  if (first < 2) {
+    const uint32_t mb = map_bit(first == 0 ? pr.x : pr.y);
+    atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
+  }
+}
+
+// After an accepted swap of species a and b: refreshes the entries that
+// changed, rows and columns a and b of every shell (4K - 4 per shell), by the
+// 64 lanes of one wave.  The bound maxima only grow: every other entry's
+// bounds are unchanged, so the maxima stay upper bounds of the current
+// entries' bounds (beta stays valid, at worst wider than needed).
+template <int KMAX>
+__device__ __forceinline__ void refresh_rows(const HeaMcArgs& A, const Lds<KMAX>& T, int a, int b,
+                                             int lane) {
+  const int K = A.K, S = A.S, per = 4 * K - 4;
+  for (int e0 = 0; e0 < S * per; e0 += 64) {
+    const int e = e0 + lane;
+    EntryBounds r{0.0, 0.0, 0.0f};
+    int s = HEA_MAX_S;
+    if (e < S * per) {
+      s = e / per;
+      const int k = e - s * per;
+      int i, j;
+      if (k < 2 * K) {  // rows a and b
+        i = k < K ? a : b;
+        j = k < K ? k : k - K;
+      } else {          // columns a and b of the other rows
+        const int m = k - 2 * K, h = m / (K - 2);
+        int y = m - h * (K - 2);
+        y += (y >= min(a, b));
+        y += (y >= max(a, b));
+        i = y;
+        j = h ? b : a;
+      }
+      r = refresh_entry_core<KMAX, true>(A, T, s, i, j);
+    }
+    for (int t = 0; t < S; ++t) {
+      const double bm = wave_max(s == t ? r.P : 0.0);
+      const double um = wave_max(s == t ? r.ub : 0.0);
+      if (lane == 0) {  // non-negative doubles order like their bit patterns
+        const unsigned long long nb = (unsigned long long)__double_as_longlong(bm);
+        const unsigned long long nu = (unsigned long long)__double_as_longlong(um);
+        if (nb > T.sh->bmax[t]) T.sh->bmax[t] = nb;
+        if (nu > T.sh->ubmax[t]) T.sh->ubmax[t] = nu;
+      }
+    }
+    const unsigned hm = __float_as_uint((float)wave_max((double)r.hz));
+    if (lane == 0 && hm > T.sh->gmax) T.sh->gmax = hm;
+  }
+}
+
+// Guided mode: the further accepts of a round, made by wave 0 alone, so an
+// accept costs no workgroup barrier.  T.sl holds the round's shortlist: up to
+// 64 proposals with predicted decreases (X < 0), the smallest X of each
+// wave's share of the round's screen.  Each step drops the proposals that
+// touch a site swapped in this round or one of its neighbours (their walks
+// are stale), screens the others again from their stored walks against the
+// refreshed tables, and sends those within 2 beta of the smallest X to the
+// exact stage, which accepts the first strict minimum iff its exact fitness
+// is lower than the current one (the rule of accelerate.cpp:663-689 applied
+// to the remaining proposals).  Stops at the first reject, when no proposal
+// is left, or after `steps` accepts.  Returns the accepts; adds the
+// proposals screened to *screened; sets *hit when an accepted fitness is
+// <= thr.
+template <int KMAX, bool EXACT>
+__device__ int guided_more_accepts(const HeaMcArgs& A, const Lds<KMAX>& T, uint8_t* sp,
+                                   unsigned long long* hist, const HeaHistLayout& hl, int lane,
+                                   int task, int steps, long long* screened, bool* hit) {
+  const int K = A.K, S = A.S;
+  int j = T.sl[lane];
+  int made = 0;
+  for (int step = 0; step < steps; ++step) {
+    if (j >= 0 && touched(T, T.prop[j])) j = -1;  // for good: its walk is stale
+    const double beta = screen_beta<KMAX>(A, T);
+    double X = __builtin_inf();
+    if (j >= 0) {
+      X = 0.0;
+      for (int s = 0; s < S; ++s) {
+        const int w = s * task + j;
+        const WalkRec r = walk_unpack(T.WC[w]);
+        X = X + A.weight[s] * screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s, r.a, r.b,
+                                                        T.WA[w], T.WB[w], r.c11, r.c12,
+                                                        r.c21, r.c22);
+      }
+      T.xs[j] = X;
+    }
+    const unsigned long long live = __ballot(j >= 0);
+    *screened += __popcll(live);
+    const double xmin = wave_min(X);
+    if (!(xmin < __builtin_inf())) break;
+    const bool f = j >= 0 && !(X > xmin + 2.0 * beta);
+    const unsigned long long bal = __ballot(f);
+    if (f) T.cl[__popcll(bal & ((1ull << lane) - 1ull))] = j;
+    wave_sync();
+    exact_stage<KMAX, EXACT>(A, T, sp, lane, task, __popcll(bal), xmin, beta, false);
+    wave_sync();
+    const int kind = T.sh->res_kind;
+    if (kind == 0) break;
+    ++made;
+    const int2 pr = T.prop[T.sh->res_j];
+    const int a = T.sh->res_a, b = T.sh->res_b;
+    if (kind == 1 && __longlong_as_double((long long)T.sh->res_key) <= A.thr) *hit = true;
+    wave_sync();
+    if (lane == 0) {
+      sp[pr.x] = (uint8_t)b;
+      sp[pr.y] = (uint8_t)a;
+    }
+    guided_apply_swap<KMAX>(A, T, hist, hl, pr, a, b, lane, 64, swap_nbrs_load(A, pr, lane, 64));
+    refresh_rows<KMAX>(A, T, a, b, lane);
+    wave_sync();
+  }
+  return made;
 }
 
 // Local search of lattice l of the population described by A.  GUIDED
@@ -839,14 +1251,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     nxt = false;
     nbal_ok = false;
 
-    // Guided mode accepts up to g_multi swaps per round: after an accept,
-    // the proposals that touch none of this round's swapped sites or their
-    // neighbours keep valid walks and are screened again against the
-    // refreshed tables; the best goes through the exact stage.
+    // Guided mode accepts up to g_multi swaps per round: the first as in
+    // plain mode, the others from the round's shortlist (guided_more_accepts).
     if constexpr (GUIDED)
       for (int e = tid; e < kMapWords; e += nthr) T.cmap[e] = 0u;
-    for (int sub = 0;; ++sub) {
-    ++screens;
+    screens += task;
     // window of the current tables and an empty candidate list, set by the
     // first lane of the shell with the fewest neighbours (its waves finish the
     // screen first); the screen's barrier publishes both
@@ -861,14 +1270,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       const int j = jl + q * TP;
       const int2 pr = (j < task) ? T.prop[j] : make_int2(-1, -1);
       double x = 0.0;
-      if (GUIDED && sub > 0) {
-        if (part == 0 && pr.x >= 0 && !touched(T, pr)) {  // walk stored by the first screen
-          const int w = s_me * task + j;
-          const WalkRec r = walk_unpack(T.WC[w]);
-          x = screen_shell<KMAX, EXACT>(T, EXACT ? KMAX : K, s_me, r.a, r.b, T.WA[w], T.WB[w],
-                                        r.c11, r.c12, r.c21, r.c22);
-        }
-      } else {
+      {
         PT MA = 0, MB = 0;
         int c11 = 0, c12 = 0, c22 = 0, c21 = 0, a = 0, b = 0;
         if (pr.x >= 0) {
@@ -911,15 +1313,37 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     double lmin = __builtin_inf();
     double X1 = 0.0;  // this lane's X when the round fits one pass
     bool has1 = false;
+    int jmin = -1;    // guided mode: this lane's proposal with the smallest X
     for (int j = tid; j < task; j += nthr) {
       const int2 pr = T.prop[j];
-      if (pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr))) {
+      if (pr.x >= 0) {
         double X = 0.0;
         for (int s = 0; s < S; ++s) X = X + A.weight[s] * T.xs[s * task + j];
         T.xs[j] = X;
+        if (X < lmin) jmin = j;
         lmin = fmin(lmin, X);
         X1 = X;
         has1 = true;
+      }
+    }
+    if constexpr (GUIDED) {
+      // shortlist for the round's further accepts: each of the nwp waves
+      // holding proposals contributes its ceil(64 / nwp) smallest X < 0
+      const int nwp = min(nwaves, (task + 63) >> 6);
+      const int per = (64 + nwp - 1) / nwp;
+      if (A.g_multi > 1 && wave < nwp) {
+        double v = (lmin < 0.0) ? lmin : __builtin_inf();
+        for (int r = 0; r < per; ++r) {
+          const double m = wave_min(v);
+          int jr = -1;
+          if (m < 0.0) {
+            const unsigned long long bal = __ballot(v == m);
+            const int src = (int)__builtin_ctzll(bal);
+            jr = __builtin_amdgcn_readlane(jmin, src);
+            if (lane == src) v = __builtin_inf();
+          }
+          if (lane == 0 && wave * per + r < 64) T.sl[wave * per + r] = jr;
+        }
       }
     }
     lmin = wave_min(lmin);
@@ -943,7 +1367,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
           bool f = false;
           if (j < task) {
             const int2 pr = T.prop[j];
-            f = pr.x >= 0 && !(GUIDED && sub > 0 && touched(T, pr)) && !(T.xs[j] > wthr);
+            f = pr.x >= 0 && !(T.xs[j] > wthr);
           }
           const unsigned long long bal = __ballot(f);
           if (bal) {
@@ -963,12 +1387,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // the exact stage is this workgroup's critical path: issue priority over
       // the waves of other workgroups on the CU until it ends
       __builtin_amdgcn_s_setprio(2);
-      // the round's state, loaded together with the wave minima (one LDS
-      // round trip; the fences below would keep later loads from moving up)
       const double beta = T.sh->beta;
       const int nsup = T.sh->ncl;
-      double prev_fit = T.sh->pf, fest = T.sh->fest;
-      int pfm = T.sh->pfm;
       double xmin = __builtin_inf();
 #pragma unroll
       for (int w2 = 0; w2 < kMaxWaves / 2; ++w2) {
@@ -976,214 +1396,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         const double2 m2 = reinterpret_cast<const double2*>(T.sh->wmin)[w2];
         xmin = fmin(xmin, fmin(m2.x, m2.y));
       }
-      double thr = xmin + 2.0 * beta;
-      if (!(thr < __builtin_inf())) thr = __builtin_inf();
-      // filter the waves' lists against the workgroup's window, compacting
-      // in place (every entry is read before any of its chunk is written)
-      int nc = 0, jfirst = -1;  // jfirst = cl[0] after the compaction
-      for (int k0 = 0; k0 < nsup; k0 += 64) {
-        const int j = (k0 + lane < nsup) ? T.cl[k0 + lane] : -1;
-        const bool f = j >= 0 && !(T.xs[j] > thr);
-        const unsigned long long bal = __ballot(f);
-        if (nc == 0 && bal) jfirst = __builtin_amdgcn_readlane(j, (int)__builtin_ctzll(bal));
-        wave_sync();
-        if (f) T.cl[nc + __popcll(bal & ((1ull << lane) - 1ull))] = j;
-        nc += __popcll(bal);
-      }
-      wave_sync();
-      // lane (c, s, x): candidate c of the chunk, shell s, species x
-      const int GS = S * K;      // lanes per candidate
-      const int cpc = 64 / GS;   // candidates per chunk
-      const int g = lane / K, x = lane - g * K;
-      const int c = g / S, s = g - c * S;
-      constexpr int XW = 4 + KMAX;
-      double* slot = T.xw + lane * XW;
-      const double* grp = T.xw + g * K * XW;  // lane (c, s, y) at grp + y * XW
-      unsigned long long bk = ~0ull;
-      int bj = 0x7fffffff, bc = 0, wc0 = -1;
-      int ra = 0, rb = 0;  // window-decided accept: the winner's species (lanes < S*K)
-      // When prev_fit = F(gamma), the window decides most rounds: every
-      // proposal's err lies within beta of E + X (E common, X = 0 for the
-      // current lattice), and 2 beta keeps sqrt's rounding out of the way.
-      // One candidate with X < -2 beta is certainly accepted; X > 2 beta for
-      // the smallest X is certainly a reject.  Near the threshold, or
-      // otherwise, the exact path decides.
-      const double mg = 2.0 * beta;
-      const bool near = A.thr > 0.0 && !(fest + xmin > A.thr * A.thr * 1.01);
-      int kind = -1;
-      if (pfm != 0 && !near) {
-        if (nc == 1 && xmin + mg < 0.0) kind = 2;
-        else if (xmin - mg > 0.0) kind = 0;
-      }
-      if (kind == -1 && pfm == 2) {  // the exact path compares with prev_fit
-        prev_fit = current_fitness<KMAX, EXACT>(A, T, lane);
-        pfm = 1;
-      }
-      if (kind == 0) {
-        if (lane == 0) T.sh->res_kind = 0;
-      } else {
-      for (int c0 = 0; c0 < nc; c0 += cpc) {
-        int j = -1;
-        if (nc == 1)  // the usual case: the candidate came with the filter's ballot
-          j = (lane < GS) ? jfirst : -1;
-        else if (lane < cpc * GS && c0 + c < nc)
-          j = T.cl[c0 + c];
-        int a = 0, b = 0;
-        double nA = 0.0, nB = 0.0;
-        if (j >= 0) {
-          const int2 pr = T.prop[j];  // only the rare replay path reads the sites
-          const int w = s * task + j;
-          const WalkRec r = walk_unpack(T.WC[w]);
-          a = r.a;
-          b = r.b;
-          exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w], r.c11, r.c12,
-                              r.c21, r.c22, nA, nB);
-          slot[0] = nA;
-          slot[1] = nB;
-          if (kind != 2) {  // alphas feed the terms, which a decided accept skips
-            slot[2] = 1.0 - nA / (T.conc[a] * T.conc[x]);
-            slot[3] = 1.0 - nB / (T.conc[b] * T.conc[x]);
-          }
-        }
-        wave_sync();
-        if (kind == 2) {  // the one candidate is accepted: its new rows only
-          if (j >= 0) {
-            T.G[T.idx(s, a, x)] = nA;
-            T.G[T.idx(s, x, a)] = nA;
-            T.G[T.idx(s, b, x)] = nB;
-            T.G[T.idx(s, x, b)] = nB;
-          }
-          bj = jfirst;
-          ra = a;
-          rb = b;
-          break;
-        }
-        if (j >= 0) {  // terms of row x (accelerate.cpp:560-566)
-          const double* tg = T.tgt + T.idx(s, x, 0);
-          const double* e0 = T.E0 + T.idx(s, x, 0);
-          const double sAx = slot[2], sBx = slot[3];
-#pragma unroll
-          for (int y = 0; y < KMAX; ++y) {
-            if (EXACT || y < K) {
-              const bool chg = (x == a) || (x == b) || (y == a) || (y == b);
-              const double sro = (x == a)   ? grp[y * XW + 2]
-                                 : (x == b) ? grp[y * XW + 3]
-                                 : (y == a) ? sAx
-                                            : sBx;
-              const double df = sro - tg[y];
-              slot[4 + y] = chg ? df * df : e0[y];
-            }
-          }
-        }
-        wave_sync();
-        if (j >= 0 && x == 0) {  // the shell's K*K terms in row-major order
-          double es = 0.0;
-#pragma unroll
-          for (int i = 0; i < KMAX; ++i)
-#pragma unroll
-            for (int y = 0; y < KMAX; ++y)
-              if (EXACT || (i < K && y < K)) es = es + grp[i * XW + 4 + y];
-          T.ex[g] = es;
-        }
-        wave_sync();
-        unsigned long long rk = ~0ull;
-        int rj = 0x7fffffff, rc = 0;
-        if (j >= 0 && s == 0 && x == 0) {
-          double err = 0.0;
-          for (int t = 0; t < S; ++t) err = err + A.weight[t] * T.ex[c * S + t];
-          rk = (unsigned long long)__double_as_longlong(sqrt(err));
-          rj = j;
-          rc = c;
-        }
-        if (nc - c0 == 1) {  // the usual case: one candidate, in lane 0
-          rk = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(rk >> 32)) << 32) |
-               (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)rk);
-          rj = __builtin_amdgcn_readfirstlane(rj);
-          rc = 0;
-        } else {
-#pragma unroll
-          for (int m = 32; m >= 1; m >>= 1) {
-            const unsigned long long ok =
-                ((unsigned long long)(uint32_t)__shfl_xor((int)(rk >> 32), m) << 32) |
-                (uint32_t)__shfl_xor((int)(uint32_t)rk, m);
-            const int oj = __shfl_xor(rj, m);
-            const int oc = __shfl_xor(rc, m);
-            if (key_less(ok, oj, rk, rj)) {
-              rk = ok;
-              rj = oj;
-              rc = oc;
-            }
-          }
-        }
-        if (key_less(rk, rj, bk, bj)) {
-          bk = rk;
-          bj = rj;
-          bc = rc;
-          wc0 = c0;
-        }
-        if (c0 + cpc < nc) wave_sync();  // the next chunk reuses xw and ex
-      }
-      if (kind == 2) {
-        if (lane < HEA_MAX_S) {  // the refresh below takes new maxima
-          T.sh->bmax[lane] = 0ull;
-          T.sh->ubmax[lane] = 0ull;
-        }
-        if (lane == 0) {
-          T.sh->gmax = 0u;
-          T.sh->res_kind = 2;
-          T.sh->res_j = bj;
-          T.sh->res_a = ra;
-          T.sh->res_b = rb;
-          T.sh->pfm = 2;
-          T.sh->fest = fest + xmin;
-        }
-      } else {
-      const double best_fit = __longlong_as_double((long long)bk);
-      int a = 0, b = 0;
-      if (best_fit < prev_fit) {  // accept: write the winner's rows
-        const int2 pr = T.prop[bj];
-        a = sp[pr.x];
-        b = sp[pr.y];
-        if (wc0 + cpc < nc) {  // a later chunk overwrote the winner's entries
-          if (lane < GS) {
-            const int w = s * task + bj;
-            const WalkRec r = walk_unpack(T.WC[w]);
-            double nA, nB;
-            exact_entries<KMAX>(A, T, sp, s, pr.x, pr.y, a, b, x, T.WA[w], T.WB[w], r.c11,
-                                r.c12, r.c21, r.c22, nA, nB);
-            slot[0] = nA;
-            slot[1] = nB;
-          }
-          bc = 0;
-        }
-        wave_sync();
-        // rows a and b of every shell, with their mirrors (columns a and b)
-        for (int e = lane; e < S * 2 * K; e += 64) {
-          const int t = e / (2 * K), h = (e / K) & 1, y = e % K;
-          const double v = T.xw[((bc * S + t) * K + y) * XW + h];
-          const int r = h ? b : a;
-          T.G[T.idx(t, r, y)] = v;
-          T.G[T.idx(t, y, r)] = v;
-        }
-        if (lane < HEA_MAX_S) {  // the refresh below takes new maxima
-          T.sh->bmax[lane] = 0ull;
-          T.sh->ubmax[lane] = 0ull;
-        }
-        if (lane == 0) T.sh->gmax = 0u;
-      }
-      if (lane == 0) {
-        const bool acc1 = best_fit < prev_fit;
-        T.sh->res_kind = acc1 ? 1 : 0;
-        T.sh->res_key = bk;
-        T.sh->res_j = bj;
-        T.sh->res_a = a;
-        T.sh->res_b = b;
-        T.sh->pf = acc1 ? best_fit : prev_fit;
-        T.sh->pfm = acc1 ? 1 : pfm;
-        if (acc1) T.sh->fest = best_fit * best_fit;
-      }
-      }
-      }
+      exact_stage<KMAX, EXACT>(A, T, sp, lane, task, nsup, xmin, beta, true);
       __builtin_amdgcn_s_setprio(0);
     } else if constexpr (!GUIDED) {
       hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pos + (uint32_t)tid, (uint32_t)N, &nx1,
@@ -1197,7 +1410,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     }
     __syncthreads();
     const int kind = T.sh->res_kind;
-    if (sub == 0) ++rounds;
+    ++rounds;
     if (!GUIDED && nthr > 64) {
       if (wave == 0) {
         const int2 v = dq[lane];
@@ -1225,29 +1438,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         q1 = sp[nx1];
         q2 = sp[nx2];
       }
-      if constexpr (GUIDED) {
-        // neighbours of the swapped sites see a -> b (around pr.x) and
-        // b -> a (around pr.y) in their histograms
-        int zs = 0;
-        for (int s = 0; s < S; ++s) zs += A.Z[s];
-        for (int t = tid; t < 2 * zs; t += nthr) {
-          const int site = t < zs ? pr.x : pr.y;
-          int q = t < zs ? t : t - zs, s = 0;
-          while (q >= A.Z[s]) q -= A.Z[s++];
-          const int nb = A.nbr[(size_t)site * A.stride + A.off[s] + q];
-          // the last species has no field (its count is implied)
-          const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + hl.o[s]) : 0ull;
-          const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + hl.o[s]) : 0ull;
-          const unsigned long long dv = t < zs ? fb - fa : fa - fb;  // modulo 2^64
-          __hip_atomic_fetch_add(hist + nb, dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const uint32_t mb = map_bit(nb);
-          atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
-        }
-        if (tid < 2) {
-          const uint32_t mb = map_bit(tid == 0 ? pr.x : pr.y);
-          atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
-        }
-      }
+      if constexpr (GUIDED)
+        guided_apply_swap<KMAX>(A, T, hist, hl, pr, a, b, tid, nthr,
+                                swap_nbrs_load(A, pr, tid, nthr));
       for (int e = tid; e < S * K * K; e += nthr) {
         const int s = e / (K * K), i = (e / K) % K, j = e % K;
         refresh_entry<KMAX, GUIDED>(A, T, s, i, j);
@@ -1261,15 +1454,20 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         nbal_ok = true;
       }
       __syncthreads();
+      if constexpr (GUIDED) {
+        if (A.g_multi > 1) {
+          if (wave == 0)
+            accepts += guided_more_accepts<KMAX, EXACT>(A, T, sp, hist, hl, lane, task,
+                                                        A.g_multi - 1, &screens, &hit_thr);
+          __syncthreads();
+        }
+      }
       if (hit_thr && tid == 0 && A.first_hit) {
         atomicMin(A.first_hit, (unsigned long long)wall_clock64());
         hit_thr = false;
       }
-      if (!GUIDED || sub + 1 >= A.g_multi) break;
     } else {
-      if (sub == 0) ++acc;
-      break;
-    }
+      ++acc;
     }
   }
 
@@ -1291,7 +1489,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     int64_t* st = A.stats + (size_t)l * 4;
     st[0] += rounds;
     st[1] += accepts;
-    st[2] += screens * (long long)task;
+    st[2] += screens;
     st[3] += (long long)pos;
   }
 }
