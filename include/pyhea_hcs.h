@@ -176,6 +176,18 @@ int hea_session_tile(hea_session s, int32_t tile);
  * shells of the bit width of Z_s <= 64 (FCC 12/6/24: 12 bits per species).
  * Call before hea_session_init/step. */
 int hea_session_guide(hea_session s, int32_t candidates, double tau, int32_t accepts);
+/* Test capture with no counterpart in the reference: the first candidate
+ * phase of a guided local search (as hea_session_guide describes it) on each
+ * of the caller's lattices (host [solutions][N]), streams (seed, iter, phase,
+ * slot = position).  out_pairs [solutions][parallel_task * candidates][2]
+ * receives every scored candidate pair (i, j), out_pred
+ * [solutions][parallel_task * candidates] its predicted first-order change of
+ * the fitness squared, u[i][b] + u[j][a] with a, b the species of i and j and
+ * u the site gains of hea_site_gain at the lattice's dn = dF^2/dn (NaN for
+ * same-species pairs).  No search step is taken.  Synchronous. */
+int hea_guided_scores(hea_problem p, int32_t solutions, const uint8_t* lattices,
+                      int32_t parallel_task, int32_t candidates, uint64_t seed, uint32_t iter,
+                      uint32_t phase, int32_t* out_pairs, float* out_pred);
 /* Runs `n_iter` global iterations asynchronously on the session stream. */
 int hea_session_step(hea_session s, int32_t n_iter);
 /* Measurement helper with no counterpart in the reference: runs n_iter

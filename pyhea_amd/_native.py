@@ -34,6 +34,7 @@ EXPORTS = (
     "hea_session_init",
     "hea_session_tile",
     "hea_session_guide",
+    "hea_guided_scores",
     "hea_session_step",
     "hea_session_step_timed",
     "hea_session_sync",
@@ -114,6 +115,8 @@ def _declare(lib: C.CDLL) -> None:
         "hea_session_init": (C.c_int, [P]),
         "hea_session_tile": (C.c_int, [P, I32]),
         "hea_session_guide": (C.c_int, [P, I32, D, I32]),
+        "hea_guided_scores": (C.c_int, [P, I32, u8p, I32, I32, U64, U32, U32, i32p,
+                                        C.POINTER(C.c_float)]),
         "hea_session_step": (C.c_int, [P, I32]),
         "hea_session_step_timed": (C.c_int, [P, I32, C.POINTER(C.c_float),
                                              C.POINTER(C.c_int64)]),

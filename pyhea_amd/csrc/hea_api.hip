@@ -341,6 +341,8 @@ struct GuideCfg {
   int cand = 0;
   double tau = 0.0;
   int multi = 1;
+  int32_t* dbg_pairs = nullptr;  // test capture of the first candidate phase
+  float* dbg_pred = nullptr;
 };
 
 }  // namespace
@@ -553,6 +555,8 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.g_cand = gd.cand;
   a.g_tau = gd.tau;
   a.g_multi = gd.multi;
+  a.dbg_pairs = gd.dbg_pairs;
+  a.dbg_pred = gd.dbg_pred;
   const HeaConst& c = p->hc;
   a.N = c.N;
   a.K = c.K;
@@ -1026,6 +1030,47 @@ static int check_guide(hea_problem p, int candidates, double tau, int accepts) {
   const HeaHistLayout h = hea_hist_layout(p->hc.S, p->hc.Z);
   if ((p->hc.K - 1) * h.W > 64)
     return fail(HEA_EUNSUPPORTED, "guided proposals need (K-1) * sum_s bitwidth(Z_s) <= 64");
+  return HEA_OK;
+}
+
+int hea_guided_scores(hea_problem p, int32_t L, const uint8_t* lattices, int32_t task,
+                      int32_t candidates, uint64_t seed, uint32_t iter, uint32_t phase,
+                      int32_t* out_pairs, float* out_pred) {
+  int rc = check_run_args(p, L, task, 1);
+  if (rc) return rc;
+  if ((rc = check_guide(p, candidates, 0.1, 1))) return rc;
+  if (!lattices || !out_pairs || !out_pred) return fail(HEA_EINVAL, "null buffer");
+  HEA_HIP(hipSetDevice(p->device));
+  if ((rc = ensure_scratch(p, L))) return rc;
+  const size_t n = (size_t)L * p->hc.N, items = (size_t)L * task * candidates;
+  DevBuf<unsigned long long> hist;
+  DevBuf<int32_t> pairs;
+  DevBuf<float> pred;
+  HEA_HIP(hist.alloc(n));
+  HEA_HIP(pairs.alloc(2 * items));
+  HEA_HIP(pred.alloc(items));
+  HEA_HIP(hipMemcpy(p->d_lat.p, lattices, n, hipMemcpyHostToDevice));
+  if ((rc = launch_recount(p, p->d_lat.p, p->d_counts.p, L, nullptr))) return rc;
+  if ((rc = launch_fitness(p, p->d_counts.p, p->d_fit.p, L, nullptr))) return rc;
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, nullptr, p->d_iter.p, iter);
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, nullptr,
+                     reinterpret_cast<int32_t*>(p->d_stats.p), (size_t)8 * L);
+  size_t lds;
+  bool sp_lds;
+  if ((rc = mc_lds(p, task, &lds, &sp_lds))) return rc;
+  GuideCfg g;
+  g.hist = hist.p;
+  g.cand = candidates;
+  g.tau = 0.1;
+  g.multi = 1;
+  g.dbg_pairs = pairs.p;
+  g.dbg_pred = pred.p;
+  if ((rc = launch_mc(p, p->d_lat.p, p->d_fit.p, p->d_counts.p, p->d_stats.p, p->d_iter.p,
+                      nullptr, L, task, 1, seed, phase, -1.0, lds, sp_lds, nullptr, g)))
+    return rc;
+  HEA_HIP(hipDeviceSynchronize());
+  HEA_HIP(hipMemcpy(out_pairs, pairs.p, sizeof(int32_t) * 2 * items, hipMemcpyDeviceToHost));
+  HEA_HIP(hipMemcpy(out_pred, pred.p, sizeof(float) * items, hipMemcpyDeviceToHost));
   return HEA_OK;
 }
 

@@ -62,6 +62,12 @@ struct HeaMcArgs {
   int32_t g_cand;               // candidate pairs per proposal
   double g_tau;                 // softmax temperature in units of the gain scale
   int32_t g_multi;              // accepts per round at most
+  // test capture of the first candidate phase (nullptr: none): when set, the
+  // search records every (slot, candidate) pair [L][task*g_cand][2] and its
+  // predicted change of F^2 [L][task*g_cand] (NaN for same-species pairs),
+  // then returns without searching
+  int32_t* dbg_pairs;
+  float* dbg_pred;
   // problem scalars, copied from HeaConst so they live in the kernarg segment
   int32_t N, K, S, stride;
   int32_t Z[HEA_MAX_S], off[HEA_MAX_S];

@@ -835,6 +835,36 @@ __device__ __forceinline__ void exact_stage(const HeaMcArgs& A, const Lds<KMAX>&
   }
 }
 
+// Guided mode: first-order change of F^2 when site i (species a, neighbour
+// record ri) and site j (species b, record rj) swap species:
+// sum_s sum_d (H_s[b][d] - H_s[a][d]) (cnt_s[i][d] - cnt_s[j][d]) = u[i][b] +
+// u[j][a] in the site gains u of the relaxation's gradient.  Absent shells
+// and species have zero rows in H; the last species' count difference is
+// minus the others' (both lists hold Z_s).
+template <int KMAX>
+__device__ __forceinline__ float guided_gain(const Lds<KMAX>& T, const HeaHistLayout& hl, int K,
+                                             int a, int b, unsigned long long ri,
+                                             unsigned long long rj) {
+  float d = 0.0f;
+#pragma unroll
+  for (int sh = 0; sh < HEA_MAX_S; ++sh) {
+    const float* Ha = T.H + T.hidx(sh, a, 0);
+    const float* Hb = T.H + T.hidx(sh, b, 0);
+    const unsigned long long fm = (1ull << hl.w[sh]) - 1ull;
+    int rest = 0;
+#pragma unroll
+    for (int x = 0; x < KMAX - 1; ++x) {
+      const int sft = min(x * hl.W + hl.o[sh], 63);  // in range; x >= K-1 is masked
+      const int dc = (x < K - 1) ? (int)((ri >> sft) & fm) - (int)((rj >> sft) & fm) : 0;
+      rest -= dc;
+      d += (Hb[x] - Ha[x]) * (float)dc;
+    }
+    const float* Hl = Hb + (K - 1);  // species K-1
+    d += (*Hl - Ha[K - 1]) * (float)rest;
+  }
+  return d;
+}
+
 // Guided mode: the neighbours of a swap's two sites, work item t of
 // [0, 2 sum Z) (a1's lists, then a2's) in lane t mod stride, slot t / stride;
 // -1 where there is no item.  2 sum Z <= 186 <= 3 x 64, so three slots cover
@@ -1131,6 +1161,19 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       const float gs = 2.0f * __uint_as_float(T.sh->gmax);
       const float inv = (float)(1.0 / (A.g_tau * (gs > 0.0f ? (double)gs : 1.0)));
       const int g = A.g_cand, items = task * g;
+      if (A.dbg_pred) {  // test capture of the first candidate phase (see HeaMcArgs)
+        for (int w = tid; w < items; w += nthr) {
+          uint32_t s1, s2, h3;
+          guided_pair(gk, pos + (uint32_t)w, (uint32_t)N, &s1, &s2, &h3);
+          const int a = sp[s1], b = sp[s2];
+          const size_t o = (size_t)l * items + w;
+          A.dbg_pred[o] = a != b ? guided_gain<KMAX>(T, hl, K, a, b, hist[s1], hist[s2])
+                                 : __builtin_nanf("");
+          A.dbg_pairs[2 * o] = (int)s1;
+          A.dbg_pairs[2 * o + 1] = (int)s2;
+        }
+        return;  // no search step: the lattice and its fitness stay as they were
+      }
       // four items per lane at a time, so their histogram loads overlap
       for (int w0 = tid; w0 < items; w0 += 4 * nthr) {
         uint32_t s1[4], s2[4], h3[4];
@@ -1154,27 +1197,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int a = sa[q], b = sb[q];
-          // first-order change of F^2: sum_s sum_d (H_s[b][d] - H_s[a][d]) (cnt1 - cnt2);
-          // absent shells and species have zero rows; the last species'
-          // count difference is minus the others' (both lists hold Z_s)
-          float d = 0.0f;
-#pragma unroll
-          for (int sh = 0; sh < HEA_MAX_S; ++sh) {
-            const float* Ha = T.H + T.hidx(sh, a, 0);
-            const float* Hb = T.H + T.hidx(sh, b, 0);
-            const unsigned long long fm = (1ull << hl.w[sh]) - 1ull;
-            int rest = 0;
-#pragma unroll
-            for (int x = 0; x < KMAX - 1; ++x) {
-              const int sft = min(x * hl.W + hl.o[sh], 63);  // in range; x >= K-1 is masked
-              const int dc = (x < K - 1) ? (int)((hv[q][0] >> sft) & fm) - (int)((hv[q][1] >> sft) & fm) : 0;
-              rest -= dc;
-              d += (Hb[x] - Ha[x]) * (float)dc;
-            }
-            const float* Hl = Hb + (K - 1);  // species K-1
-            d += (*Hl - Ha[K - 1]) * (float)rest;
-          }
+          const float d = guided_gain<KMAX>(T, hl, K, sa[q], sb[q], hv[q][0], hv[q][1]);
           const float U = ((float)(h3[q] >> 8) + 0.5f) * 0x1p-24f;
           const float sc = -d * inv - __logf(-__logf(U));
           // order-preserving float bits; ties go to the lower candidate

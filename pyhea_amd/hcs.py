@@ -117,6 +117,19 @@ class Problem:
                                     thr, seed, _ptr(lat, _u8p), _ptr(fit, _dp), C.byref(st)))
         return lat, fit, st.as_dict()
 
+    def guided_scores(self, lattices, parallel_task: int, candidates: int, seed: int,
+                      iter: int = 0, phase: int = N.PHASE_LS1):
+        """The guided search's first candidate phase on each lattice:
+        (pairs [L, T*cand, 2], predicted change of F^2 [L, T*cand])."""
+        lat = self._lat(lattices)
+        L, items = lat.shape[0], parallel_task * candidates
+        pairs = np.empty((L, items, 2), dtype=np.int32)
+        pred = np.empty((L, items), dtype=np.float32)
+        N.check(N.lib().hea_guided_scores(self._h, L, _ptr(lat, _u8p), parallel_task, candidates,
+                                          seed, iter, phase, _ptr(pairs, _i32p),
+                                          pred.ctypes.data_as(C.POINTER(C.c_float))))
+        return pairs, pred
+
     def session(self, solutions: int, parallel_task: int, converge_depth: int,
                 thr: float = 1e-3, seed: int = 0x5EED) -> "Session":
         return Session(self, solutions, parallel_task, converge_depth, thr, seed)

@@ -102,3 +102,37 @@ def test_guided_uniform_species_batch_matches_sessions(gpu, K):
         ls, fs, _ = s.read()
         lb, fb, _ = b.read(q)
         assert np.array_equal(lb, ls) and np.array_equal(fb, fs)
+
+
+@pytest.mark.parametrize("typ,cell,K", [("FCC", 4, 4), ("BCC", 5, 5), ("FCC", 3, 6)])
+def test_guided_scores_are_relaxation_site_gains(gpu, typ, cell, K):
+    """The guided kernel's predicted change of F^2 for a candidate swap
+    (i: a -> b, j: b -> a), from its neighbour-count records and gain table,
+    equals u[i][b] + u[j][a] of the relaxation's site gains (hea_site_gain)
+    at the same lattice with dn = dF^2/dn (the relaxation's gradient at the
+    one-hot state): the coupling that ranks and samples the proposals."""
+    import torch
+
+    from pyhea_amd.relax import RelaxObjective, site_gain
+
+    spec = _spec(typ, cell, K)
+    prob = Problem.from_spec(spec)
+    lat, _ = prob.generate(3, seed=17)
+    pairs, pred = prob.guided_scores(lat, 64, 8, seed=23)
+    n = torch.as_tensor(prob.pair_counts(lat), dtype=torch.float64, device="cuda")
+    dn = RelaxObjective(prob).count_gradient(n)
+    u = site_gain(prob, torch.as_tensor(lat, device="cuda"), dn).cpu().numpy().astype(np.float64)
+    i, j = pairs[..., 0], pairs[..., 1]
+    assert i.min() >= 0 and max(i.max(), j.max()) < spec.n_sites
+    ll = np.arange(lat.shape[0])[:, None]
+    a, b = lat[ll, i], lat[ll, j]
+    live = a != b
+    assert np.array_equal(np.isnan(pred), ~live) and live.mean() > 0.5
+    want = u[ll, i, b] + u[ll, j, a]
+    # float32 gains on both sides (the kernel's table is the FP64 gradient
+    # rounded to float32, summed over 3 shells x (K-1) species)
+    scale = np.abs(u).max()
+    assert np.allclose(pred[live], want[live], rtol=1e-4, atol=1e-5 * scale)
+    # the capture leaves the lattices alone and is reproducible
+    pairs2, pred2 = prob.guided_scores(lat, 64, 8, seed=23)
+    assert np.array_equal(pairs, pairs2) and np.array_equal(pred, pred2, equal_nan=True)
