@@ -79,7 +79,7 @@ void recount_gamma(const Problem& P, const uint8_t* lat, Gamma& G) {
 // accelerate.cpp:556-569 (fitness tail of calculate_fitness_incremental) and
 // accelerate.cpp:166-178 (calculate_fitness_of_solutions, whose
 // std::pow(x, 2) the compiler folds to x*x; pinned by the golden vectors).
-double fitness_of_gamma(const Problem& P, const Gamma& G) {
+double error_of_gamma(const Problem& P, const Gamma& G) {
   double error = 0.0;
   for (int s = 0; s < P.S; ++s) {
     const double w = P.weight[s];
@@ -95,8 +95,10 @@ double fitness_of_gamma(const Problem& P, const Gamma& G) {
     }
     error += w * es;
   }
-  return sqrt(error);
+  return error;
 }
+
+double fitness_of_gamma(const Problem& P, const Gamma& G) { return sqrt(error_of_gamma(P, G)); }
 
 // accelerate.cpp:498-570 (calculate_fitness_incremental).  Mutates gamma and
 // leaves `lat` with a1/a2 swapped.  Note the a2 loop bound is a1's shell
