@@ -251,6 +251,10 @@ int hea_batch_destroy(hea_batch b);
  * be captured into a graph. */
 int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
                       void* d_n, void* stream);
+/* KP, the padded column count of the p and q rows above, for n_species
+ * species (4 for K <= 4, else 8); -1 outside 1..8.  Size the buffers with it:
+ * p holds N * KP elements, q holds S * N * KP. */
+int32_t hea_relax_cols(int32_t n_species);
 /* Neighbour-gather adjoint: d loss / d logits from dn = d loss / d n. */
 int hea_relax_backward(hea_problem p, int32_t dtype, const void* d_p, const void* d_q,
                        const void* d_dn, void* d_grad, void* stream);

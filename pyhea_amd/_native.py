@@ -53,6 +53,7 @@ EXPORTS = (
     "hea_batch_destroy",
     "hea_relax_forward",
     "hea_relax_backward",
+    "hea_relax_cols",
     "hea_site_gain",
 )
 
@@ -137,6 +138,7 @@ def _declare(lib: C.CDLL) -> None:
         "hea_batch_destroy": (C.c_int, [P]),
         "hea_relax_forward": (C.c_int, [P, I32, P, P, P, P, P]),
         "hea_relax_backward": (C.c_int, [P, I32, P, P, P, P, P]),
+        "hea_relax_cols": (I32, [I32]),
         "hea_site_gain": (C.c_int, [P, I32, P, P, P, P]),
     }
     for name, (res, args) in sig.items():

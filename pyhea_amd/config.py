@@ -101,7 +101,10 @@ class input_config:  # noqa: N801  (reference class name)
     total_iter = property(lambda self: self.config.get("total_iter", 0))
     weight = property(lambda self: self.config.get("weight", []))
     device = property(lambda self: self.config.get("device", "cpu"))
-    seed = property(lambda self: int(self.config.get("seed", 0x5EED)))
+    # None (no `seed` key): the engine draws a fresh key per run, as the
+    # reference seeds from std::random_device (pyhea/cpp/accelerate.cpp:241-242)
+    seed = property(lambda self: None if self.config.get("seed") is None
+                    else int(self.config["seed"]))
     shells = property(lambda self: self.config.get("shells", "pyhea"))
     mode = property(lambda self: self.config.get("mode", "plain"))
 
@@ -128,8 +131,8 @@ class input_config:  # noqa: N801  (reference class name)
         path = self.config.get("target_sro")
         if path is None:
             return np.zeros((3, K * K))
-        if not os.path.isabs(path):
-            path = os.path.join(os.path.dirname(os.path.abspath(self.file_path)), path)
+        # relative to the working directory, as pyhea/io/input.py:373-374 does
+        path = os.path.join(os.getcwd(), path)
         if not os.path.exists(path):
             raise FileNotFoundError(f"Target SRO file not found: {path}")
         return read_target_sro(path, K)

@@ -11,8 +11,9 @@
 //   * both run names execute the MI355X engine; it reproduces the reference
 //     CPU engine's doubles bit for bit on the same keyed streams, so the CPU
 //     name keeps its meaning;
-//   * keyword-only `seed` (the reference's streams are unseeded) and
-//     `device` (GPU ordinal);
+//   * keyword-only `seed` and `device` (GPU ordinal); without a seed every
+//     call draws one from std::random_device, as the reference seeds its
+//     permutations (accelerate.cpp:241-242), so repeated calls differ;
 //   * `nbor` may also be a 2-D integer numpy array [N][sum Z_s] with the
 //     keyword `shell_size`, which avoids building nested lists at scale;
 //   * errors raise ValueError / RuntimeError instead of exiting the process
@@ -22,6 +23,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -33,7 +35,6 @@ namespace py = pybind11;
 
 namespace {
 
-constexpr uint64_t kDefaultSeed = 0x5eedULL;
 
 void check(int rc) {
   if (rc == HEA_OK) return;
@@ -92,7 +93,15 @@ Flat flatten(const py::object& nbor, int n_shells, const py::object& shell_size)
 std::tuple<std::vector<std::vector<int>>, std::vector<double>> run(
     int nnet, int step, int task, int depth, double thr, const py::object& nbor,
     const std::vector<int>& element, const std::vector<double>& weight,
-    const py::object& target_sro, uint64_t seed, int device, const py::object& shell_size) {
+    const py::object& target_sro, const py::object& seed_arg, int device,
+    const py::object& shell_size) {
+  uint64_t seed;
+  if (seed_arg.is_none()) {
+    std::random_device rd;  // a fresh key per call (accelerate.cpp:241-242)
+    seed = ((uint64_t)rd() << 32) | rd();
+  } else {
+    seed = seed_arg.cast<uint64_t>();
+  }
   const int S = (int)weight.size();
   if (S < 1 || S > 3) throw py::value_error("len(weight) must be 1..3");
   const int K = (int)element.size();
@@ -156,8 +165,10 @@ const char* kDoc = R"pbdoc(
         Shell weights; len(weight) selects the number of shells.
     target_sro : array-like [>= len(weight)][K*K]
         Target Warren-Cowley parameters per shell.
-    seed : int, keyword-only
-        Key of every random stream of the run.
+    seed : int or None, keyword-only
+        Key of every random stream of the run; None (the default) draws a
+        fresh key from std::random_device, so calls differ like the
+        reference's.
     device : int, keyword-only
         GPU ordinal.
 
@@ -175,7 +186,7 @@ PYBIND11_MODULE(accelerate, m) {
   for (const char* name : {"run_local_parallel_hcs", "run_local_parallel_hcs_cuda"}) {
     m.def(name, &run, py::arg("nnet"), py::arg("step"), py::arg("task"), py::arg("depth"),
           py::arg("thr"), py::arg("nbor"), py::arg("element"), py::arg("weight"),
-          py::arg("target_sro"), py::kw_only(), py::arg("seed") = kDefaultSeed,
+          py::arg("target_sro"), py::kw_only(), py::arg("seed") = py::none(),
           py::arg("device") = 0, py::arg("shell_size") = py::none(), kDoc);
   }
   m.def(

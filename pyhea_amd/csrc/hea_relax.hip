@@ -55,8 +55,11 @@ __global__ void k_transpose_nbr(const int32_t* __restrict__ nbr, int N, int stri
   nbrT[t] = nbr[(size_t)i * stride + c];
 }
 
+// V elements loaded or stored together.  Alignment at most 16 bytes: the
+// entry points check 16-byte alignment of p and q, every row starts at a
+// multiple of 16 bytes, and wider vectors are moved as 16-byte pieces.
 template <typename T, int V>
-struct alignas(V * sizeof(T)) VecT {
+struct alignas(V * sizeof(T) < 16 ? V * sizeof(T) : 16) VecT {
   T x[V];
 };
 
@@ -309,7 +312,9 @@ hipError_t relax_backward_t(const HeaConst* dc, const HeaConst& hc, const void* 
 
 }  // namespace
 
-int hea_relax_cols(int K) { return relax_cols(K); }
+extern "C" int32_t hea_relax_cols(int32_t n_species) {
+  return n_species < 1 || n_species > HEA_MAX_K ? -1 : relax_cols(n_species);
+}
 
 size_t hea_relax_part_size(int n_sites, int K, int S) {
   return (size_t)relax_gather_blocks(n_sites, K) * S * K * K;

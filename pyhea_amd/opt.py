@@ -34,7 +34,7 @@ class opt_model:  # noqa: N801  (reference class name)
             raise ValueError("The number of weight is larger than the maximum number(shell) "
                              "of neighbors.")
         self.weight = conf.weight
-        self.seed = getattr(conf, "seed", 0x5EED)
+        self.seed = getattr(conf, "seed", None)  # None: a fresh key per run
 
     def run_optimization(self):
         logger.info("Running Local Parallel HCS optimization...")

@@ -35,8 +35,12 @@ def _dtype_code(t: torch.Tensor) -> int:
 
 def relax_cols(K: int) -> int:
     """Columns of the padded occupancy rows p [N][KP] and neighbour sums
-    q [S][N][KP] (zeros beyond K)."""
-    return 4 if K <= 4 else 8
+    q [S][N][KP] (zeros beyond K), as the library sizes them
+    (hea_relax_cols)."""
+    kp = int(Nat.lib().hea_relax_cols(int(K)))
+    if kp < 0:
+        raise ValueError(f"no relaxation layout for {K} species")
+    return kp
 
 
 def _stream() -> C.c_void_p:

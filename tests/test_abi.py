@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     with open(os.path.join(ROOT, "include", "pyhea_hcs.h")) as f:
         src = f.read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(hea_\w+)\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|const char\*)\s+(hea_\w+)\(", src, re.M)))
 
 
 def test_header_and_binding_agree():
@@ -114,3 +114,14 @@ def test_python_wrapper_raises_value_error(spec):
 
     with pytest.raises(ValueError):
         Problem(spec.nbr, spec.shell_sizes, [16, 8, 7], spec.weight, spec.target)
+
+
+def test_relaxation_row_width_comes_from_the_library():
+    """p and q rows are padded to 4 columns up to 4 species, else 8; the
+    library's hea_relax_cols is the one source of that rule (no GPU needed)."""
+    from pyhea_amd.relax import relax_cols
+
+    assert [relax_cols(k) for k in range(1, 9)] == [4, 4, 4, 4, 8, 8, 8, 8]
+    assert N.lib().hea_relax_cols(0) == -1 and N.lib().hea_relax_cols(9) == -1
+    with pytest.raises(ValueError):
+        relax_cols(9)

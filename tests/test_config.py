@@ -64,7 +64,8 @@ def test_random_target_sum_rule():
         assert off.min() >= -0.15 and off.max() <= 0.05
 
 
-def test_input_config_yaml(tmp_path):
+def test_input_config_yaml(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
     (tmp_path / "TARGET_SRO").write_text("Shell\n0.1\n0.2 0.3\n")
     cfg = tmp_path / "config.yaml"
     cfg.write_text("type: 2\nelement: [16, 16]\ncell_dim: [2, 2, 2]\ndevice: gpu\n"
@@ -74,6 +75,23 @@ def test_input_config_yaml(tmp_path):
     assert c.type == 2 and c.element == [16, 16] and c.device == "gpu"
     assert c.weight == [4.0, 1.0] and c.parallel_task == 8
     assert np.array_equal(c.target_sro[0], [0.1, 0.2, 0.2, 0.3])
+
+
+def test_target_sro_path_is_relative_to_the_working_directory(tmp_path, monkeypatch):
+    """A relative target_sro names a file in the working directory, not beside
+    the YAML file (pyhea/io/input.py:373-374)."""
+    ydir, wdir = tmp_path / "conf", tmp_path / "work"
+    ydir.mkdir()
+    wdir.mkdir()
+    (ydir / "TARGET_SRO").write_text("Shell\n0.1\n0.2 0.3\n")
+    (wdir / "TARGET_SRO").write_text("Shell\n0.4\n0.5 0.6\n")
+    cfg = ydir / "config.yaml"
+    cfg.write_text("type: 2\nelement: [16, 16]\ntarget_sro: TARGET_SRO\n")
+    monkeypatch.chdir(wdir)
+    assert np.array_equal(input_config(str(cfg)).target_sro[0], [0.4, 0.5, 0.5, 0.6])
+    (wdir / "TARGET_SRO").unlink()
+    with pytest.raises(FileNotFoundError):
+        input_config(str(cfg)).target_sro
 
 
 def test_input_config_rejects_unknown_key(tmp_path):

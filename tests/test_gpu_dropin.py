@@ -41,7 +41,8 @@ def test_pybind_errors_raise(gpu):
                                    np.zeros((3, 4)), shell_size=spec.shell_sizes)
 
 
-def test_yaml_to_engine_matches_reference_engine(gpu, tmp_path):
+def test_yaml_to_engine_matches_reference_engine(gpu, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)  # target_sro is relative to the working directory
     if not RefHarness.available():
         pytest.skip("reference harness not built")
     from pyhea_amd.config import input_config
@@ -60,3 +61,16 @@ def test_yaml_to_engine_matches_reference_engine(gpu, tmp_path):
     snaps, _, _ = RefHarness(o).run_hcs(4, 2, 8, 2, 77)
     assert np.array_equal(np.array(latts, np.uint8), snaps[-1][0])
     assert np.array_equal(np.array(fits), snaps[-1][1])
+
+
+def test_pybind_default_seed_is_fresh_per_call(gpu):
+    """Without `seed` every call draws a fresh stream key, as the reference
+    seeds from std::random_device (accelerate.cpp:241-242): two calls differ."""
+    from pyhea_amd.cpp import accelerate as acc
+
+    spec = make_problem("FCC", 3, [0.5, 0.3, 0.2], shells="pyhea", target_seed=3)
+    target = np.zeros((3, 9))
+    target[:3] = spec.target
+    runs = [acc.run_local_parallel_hcs(4, 1, 16, 2, 0.001, spec.nbr, spec.counts, spec.weight,
+                                       target, shell_size=spec.shell_sizes) for _ in range(2)]
+    assert not np.array_equal(np.array(runs[0][0]), np.array(runs[1][0]))
