@@ -1262,7 +1262,8 @@ struct hea_batch_s {
   DevBuf<double> fA, fB, fC;
   DevBuf<int32_t> counts, perm;
   DevBuf<int64_t> stats;
-  DevBuf<uint32_t> iter, queue;
+  DevBuf<uint32_t> iter, queue;   // queue: [HEA_XCDS] next item of each part
+  int xoff[HEA_XCDS + 1] = {};     // item parts (HeaMcBatchArgs)
   DevBuf<uint32_t> tie;            // [problems] tie flags of the population sorts
   DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
   DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
@@ -1292,10 +1293,12 @@ static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
   HeaMcBatchArgs ba;
   ba.items = b->items.p;
   ba.n_items = P * L;
+  for (int x = 0; x <= HEA_XCDS; ++x) ba.xoff[x] = b->xoff[x];
   ba.queue = b->queue.p;
   for (int phase = HEA_PHASE_LS1; phase <= HEA_PHASE_LS2; ++phase) {
     ba.probs = b->margs.p + (size_t)(phase - HEA_PHASE_LS1) * P;
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, b->queue.p, 0u);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st,
+                       reinterpret_cast<int32_t*>(b->queue.p), (size_t)HEA_XCDS);
     const int h = phase - HEA_PHASE_LS1;
     if (hk) {
       if (h == 0)
@@ -1374,7 +1377,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
     b->lds = hea_mc_lds_bytes(b->kmax, task, b->tp, maxN, false);
   }
   size_t nl = 0, nc = 0;
-  std::vector<std::pair<double, int2>> work;
+  std::vector<double> cost(n_probs);
   for (int q = 0; q < n_probs; ++q) {
     const HeaConst& c = probs[q]->hc;
     b->lat_off.push_back(nl);
@@ -1383,15 +1386,30 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
     nc += (size_t)L * c.S * c.K * c.K;
     int sz = 0;
     for (int s = 0; s < c.S; ++s) sz += c.Z[s];
-    const double cost = (double)c.N * (sz + c.K * c.K);  // rounds ~ N, round ~ sumZ + K^2
-    for (int l = 0; l < L; ++l) work.push_back({cost, make_int2(q, l)});
+    cost[q] = (double)c.N * (sz + c.K * c.K);  // rounds ~ N, round ~ sumZ + K^2
   }
-  std::stable_sort(work.begin(), work.end(),
-                   [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) {
-                     return x.first > y.first;
-                   });
+  // one part per XCD: problems, largest first, each to the least loaded
+  // part (all of a problem's lattices together, so its neighbour table is
+  // read through one L2); within a part, longest items first
+  std::vector<int> order(n_probs), part(n_probs);
+  for (int q = 0; q < n_probs; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  double load[HEA_XCDS] = {};
+  for (int q : order) {
+    int x = 0;
+    for (int k = 1; k < HEA_XCDS; ++k)
+      if (load[k] < load[x]) x = k;
+    part[q] = x;
+    load[x] += cost[q] * L;
+  }
   std::vector<int2> items;
-  for (auto& w : work) items.push_back(w.second);
+  for (int x = 0; x < HEA_XCDS; ++x) {
+    b->xoff[x] = (int)items.size();
+    for (int q : order)
+      if (part[q] == x)
+        for (int l = 0; l < L; ++l) items.push_back(make_int2(q, l));
+  }
+  b->xoff[HEA_XCDS] = (int)items.size();
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = b->A.alloc(nl);
   if (e == hipSuccess) e = b->B.alloc(nl);
@@ -1403,7 +1421,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (e == hipSuccess) e = b->perm.alloc((size_t)n_probs * 2 * L);
   if (e == hipSuccess) e = b->stats.alloc((size_t)n_probs * 4 * L);
   if (e == hipSuccess) e = b->iter.alloc(1);
-  if (e == hipSuccess) e = b->queue.alloc(1);
+  if (e == hipSuccess) e = b->queue.alloc(HEA_XCDS);
   if (e == hipSuccess) e = b->clk.alloc((size_t)1 + n_probs);
   if (e == hipSuccess) e = b->margs.alloc((size_t)2 * n_probs);
   if (e == hipSuccess) e = b->items.alloc(items.size());

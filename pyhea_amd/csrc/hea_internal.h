@@ -76,12 +76,17 @@ struct HeaMcArgs {
 
 // One launch of the persistent local search over many problems: probs[p]
 // holds problem p's complete launch arguments; items[i] = (problem,
-// lattice), processed in order.
+// lattice).  The items are split into HEA_XCDS parts, part x =
+// items[xoff[x], xoff[x+1]), one per XCD (each problem's lattices in one
+// part, longest first within a part); a workgroup takes items from its own
+// XCD's part, then from the others.
+#define HEA_XCDS 8  // XCDs of an MI355X; workgroups are dispatched round-robin over them
 struct HeaMcBatchArgs {
   const HeaMcArgs* probs;
   const int2* items;
   int32_t n_items;
-  uint32_t* queue;  // zeroed before each launch
+  int32_t xoff[HEA_XCDS + 1];
+  uint32_t* queue;  // [HEA_XCDS] next item of each part, zeroed before each launch
 };
 
 // Guided mode's per-site record: the species counts of a site's neighbours,

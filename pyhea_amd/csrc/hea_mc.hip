@@ -1311,12 +1311,29 @@ template <int KMAX, bool SP_LDS, bool GUIDED>
 __global__ __launch_bounds__(HEA_MC_MAX_THREADS, 4) void k_local_mc_batch(HeaMcBatchArgs B) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int item_sh;
+  // the part of this workgroup's XCD first (its problems' neighbour tables
+  // stay in that XCD's L2), then the others' when it runs dry; every part is
+  // tried at most once after it is found empty, so the loop ends
+  const int home = blockIdx.x % HEA_XCDS;
+  int tried = 0;  // parts found empty (thread 0)
   for (;;) {
-    if (threadIdx.x == 0) item_sh = (int)atomicAdd(B.queue, 1u);
+    if (threadIdx.x == 0) {
+      int it = -1;
+      while (tried < HEA_XCDS) {
+        const int x = (home + tried) % HEA_XCDS;
+        const uint32_t k = atomicAdd(&B.queue[x], 1u);
+        if (k < (uint32_t)(B.xoff[x + 1] - B.xoff[x])) {
+          it = B.xoff[x] + (int)k;
+          break;
+        }
+        ++tried;
+      }
+      item_sh = it;
+    }
     __syncthreads();
     const int item = __builtin_amdgcn_readfirstlane(item_sh);
     __syncthreads();
-    if (item >= B.n_items) break;
+    if (item < 0) break;
     const int2 it = B.items[item];
     const HeaMcArgs& A = B.probs[it.x];
     mc_lattice<KMAX, SP_LDS, false, false, GUIDED>(A, it.y, smem);
