@@ -242,18 +242,21 @@ int hea_batch_destroy(hea_batch b);
  * counts n[s][a][b] = sum_i sum_{j in N_s(i)} p[i][a] p[j][b] equal the integer
  * pair counts when p is one-hot.  All pointers are device memory; dtype 0 =
  * float32, 1 = float64 for logits [N][K], grad [N][K], and the occupancies
- * p [N][KP] and neighbour sums q [S][N][KP] that the forward writes for the
- * backward, rows padded with zeros to KP = 4 columns for K <= 4, else 8 (p
- * and q 16-byte aligned); n and dn are float64 [S][K][K].  Asynchronous on
+ * p (N * KP elements) and neighbour sums q (S * N * KP) that the forward
+ * writes for the backward, KP = hea_relax_cols(K) (p and q 16-byte aligned):
+ * for K <= 4 rows of 4 columns (zeros beyond K); for K > 4 columns 0-3 of
+ * every row first ([N][4]; q: [S][N][4]), then the remaining W2 = KP - 4
+ * columns ([N][W2]; q: [S][N][W2]), zeros beyond K.  n and dn are float64
+ * [S][K][K].  Asynchronous on
  * `stream`.  Calls on one problem must not overlap.  The first
  * hea_relax_forward or hea_site_gain on a problem allocates its work buffers
  * (about 4 * N * row ints bytes) and waits for `stream` once, so it must not
  * be captured into a graph. */
 int hea_relax_forward(hea_problem p, int32_t dtype, const void* d_logits, void* d_p, void* d_q,
                       void* d_n, void* stream);
-/* KP, the padded column count of the p and q rows above, for n_species
- * species (4 for K <= 4, else 8); -1 outside 1..8.  Size the buffers with it:
- * p holds N * KP elements, q holds S * N * KP. */
+/* KP, the elements of one site's p row in the layout above, for n_species
+ * species (4 for K <= 4, 5, 6, 8 for K = 5, 6, 7-8); -1 outside 1..8.  Size
+ * the buffers with it: p holds N * KP elements, q holds S * N * KP. */
 int32_t hea_relax_cols(int32_t n_species);
 /* Neighbour-gather adjoint: d loss / d logits from dn = d loss / d n. */
 int hea_relax_backward(hea_problem p, int32_t dtype, const void* d_p, const void* d_q,

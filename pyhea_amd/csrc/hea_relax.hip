@@ -13,12 +13,17 @@
 // then n_s = P^T Q_s, reduced deterministically in FP64 (a fixed-order sum
 // per block of sites, then the block partials in a fixed-shape tree).
 //
-// Layout: p [N][KP] and Q [S][N][KP] have rows padded with zeros to KP
-// columns (KP = 4 for K <= 4, else 8).  The gather gives each site KP / 4
-// lanes of four species each, so a neighbour's row is one 4-element vector
-// load per lane (16 bytes in float32), and reads the neighbour indices from
-// a slot-major copy of the table (nbrT[column][site]), one contiguous run
-// per wave; the adjoint reads Q rows and p columns as vectors too.
+// Layout of p and Q.  K <= 4: rows of 4 columns, p [N][4], Q [S][N][4]
+// (zeros beyond K).  K > 4: each row is split into part A, columns 0-3, and
+// part B, columns 4 .. 4 + W2 - 1 with W2 = 1, 2, 4 for K = 5, 6, 7-8 (zeros
+// beyond K); p holds part A of every site [N][4], then part B [N][W2]; Q
+// holds the shells' A parts [S][N][4], then their B parts [S][N][W2].  A
+// site's p row is 4 + W2 elements (hea_relax_cols), so K = 5 moves 5 values
+// per row, not 8.  The gather reads a neighbour's part as one vector load
+// (16 bytes for part A in float32), with the lanes of a wave all on the
+// same part, and reads the neighbour indices from a slot-major copy of the
+// table (nbrT[column][site]), one contiguous run per wave; the adjoint reads
+// Q and p rows as vectors too.
 // Backward (neighbour-gather adjoint): the neighbour relation is symmetric,
 // so dn[s][a][b]/dp[k][c] = [a=c] Q_s[k][b] + [b=c] Q_s[k][a] and
 //     g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q_s[k][b],
@@ -63,112 +68,135 @@ struct alignas(V * sizeof(T) < 16 ? V * sizeof(T) : 16) VecT {
   T x[V];
 };
 
-// p[i][c] = softmax(theta[i])[c] for c < K, 0 up to c < KP: KP / 4 lanes
-// per site, four columns each, written as one vector.
-template <typename T, int KP>
+// Width of part B of the rows (file header): 0 for K <= 4, else the columns
+// past the first four rounded up to 1, 2 or 4.
+constexpr int relax_w2(int K) { return K <= 4 ? 0 : K == 5 ? 1 : K == 6 ? 2 : 4; }
+
+// p = softmax(theta) in the row layout of the file header: one lane per
+// site, part A as one 4-column vector, part B (W2 columns) as another;
+// columns from K on hold 0.
+template <typename T, int W2>
 __global__ __launch_bounds__(kSoftmaxThreads) void k_softmax(const T* __restrict__ th,
                                                              T* __restrict__ p, int N, int K) {
-  constexpr int GL = KP / 4;
-  using Vec = VecT<T, 4>;
-  const int t = blockIdx.x * kSoftmaxThreads + threadIdx.x;
-  const int i = t / GL, b = t % GL;
-  if (i >= N) return;  // a site's lanes leave together
-  double x[4], m = -HUGE_VAL;
+  constexpr int C = 4 + W2;
+  const int i = blockIdx.x * kSoftmaxThreads + threadIdx.x;
+  if (i >= N) return;
+  double x[C], m = -HUGE_VAL;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int c = b * 4 + v;
-    x[v] = c < K ? (double)th[(size_t)i * K + c] : -HUGE_VAL;
-    m = fmax(m, x[v]);
+  for (int c = 0; c < C; ++c) {
+    x[c] = c < K ? (double)th[(size_t)i * K + c] : -HUGE_VAL;
+    m = fmax(m, x[c]);
   }
-  if (GL > 1) m = fmax(m, __shfl_xor(m, 1, GL));
   double z = 0.0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    x[v] = b * 4 + v < K ? exp(x[v] - m) : 0.0;
-    z += x[v];
+  for (int c = 0; c < C; ++c) {
+    x[c] = c < K ? exp(x[c] - m) : 0.0;
+    z += x[c];
   }
-  if (GL > 1) z += __shfl_xor(z, 1, GL);
-  Vec out;
+  VecT<T, 4> a;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) out.x[v] = (T)(x[v] / z);
-  *reinterpret_cast<Vec*>(p + (size_t)i * KP + b * 4) = out;
+  for (int v = 0; v < 4; ++v) a.x[v] = (T)(x[v] / z);
+  *reinterpret_cast<VecT<T, 4>*>(p + (size_t)i * 4) = a;
+  if constexpr (W2 > 0) {
+    VecT<T, W2> bb;
+#pragma unroll
+    for (int v = 0; v < W2; ++v) bb.x[v] = (T)(x[4 + v] / z);
+    *reinterpret_cast<VecT<T, W2>*>(p + (size_t)4 * N + (size_t)i * W2) = bb;
+  }
 }
 
-// Q[s][i][c] for one block of SB sites: lane b of site i sums species
-// b*V .. b*V+V-1 over i's shell-s neighbours, in list order; then the
-// block's share of n[s][a][c] = sum_i p[i][a] Q[s][i][c] from LDS: four
-// interleaved site subsets per entry, added in a fixed order.  Rows of pp
-// and Q have G*V columns; part is [S K K][nb].
-template <typename T, int G, int V>
+// One gather lane: site i's own occupancies of one row part (V columns,
+// rows of V elements from `rows`) into LDS, and for every shell the sum of
+// its neighbours' rows in list order into LDS and Q (part rows of V from
+// `qrows`, shell s at qrows + s N V).
+template <typename T, int V, int KPL>
+__device__ __forceinline__ void gather_part(const HeaConst& P, const int32_t* __restrict__ nbrT,
+                                            const T* __restrict__ rows, T* __restrict__ qrows,
+                                            int i, int SB, int il, int col0, T* sp, T* sq) {
+  using Vec = VecT<T, V>;
+  const int N = P.N, S = P.S;
+  T* myp = sp + il * KPL + col0;
+  const Vec own = *reinterpret_cast<const Vec*>(rows + (size_t)i * V);
+#pragma unroll
+  for (int v = 0; v < V; ++v) myp[v] = own.x[v];
+  for (int s = 0; s < S; ++s) {
+    const int32_t* col = nbrT + (size_t)P.off[s] * N + i;
+    const int Z = P.Z[s];
+    double acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0;
+    int k = 0;
+    for (; k + 4 <= Z; k += 4) {
+      const int j0 = col[(size_t)k * N], j1 = col[(size_t)(k + 1) * N];
+      const int j2 = col[(size_t)(k + 2) * N], j3 = col[(size_t)(k + 3) * N];
+      const Vec v0 = *reinterpret_cast<const Vec*>(rows + (size_t)j0 * V);
+      const Vec v1 = *reinterpret_cast<const Vec*>(rows + (size_t)j1 * V);
+      const Vec v2 = *reinterpret_cast<const Vec*>(rows + (size_t)j2 * V);
+      const Vec v3 = *reinterpret_cast<const Vec*>(rows + (size_t)j3 * V);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        acc[v] += (double)v0.x[v];
+        acc[v] += (double)v1.x[v];
+        acc[v] += (double)v2.x[v];
+        acc[v] += (double)v3.x[v];
+      }
+    }
+    for (; k < Z; ++k) {
+      const Vec vk = *reinterpret_cast<const Vec*>(rows + (size_t)col[(size_t)k * N] * V);
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += (double)vk.x[v];
+    }
+    Vec qv;
+#pragma unroll
+    for (int v = 0; v < V; ++v) qv.x[v] = (T)acc[v];
+    T* myq = sq + s * SB * KPL + il * KPL + col0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) myq[v] = qv.x[v];
+    *reinterpret_cast<Vec*>(qrows + ((size_t)s * N + i) * V) = qv;
+  }
+}
+
+// Q[s][i][c] for one block of SB sites, then the block's share of
+// n[s][a][c] = sum_i p[i][a] Q[s][i][c] from LDS: four interleaved site
+// subsets per entry, added in a fixed order; part is [S K K][nb].  With a
+// part B (W2 > 0) the first half of the workgroup's waves gathers part A
+// of SB = 256 sites and the second half part B of the same sites, so every
+// wave loads rows of one width.  LDS rows are KPL = 4 or 8 columns wide.
+template <typename T, int W2>
 __global__ __launch_bounds__(kGatherThreads) void k_gather_pairs(
     const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbrT, const T* __restrict__ pp,
     T* __restrict__ Q, double* __restrict__ part) {
-  constexpr int SB = kGatherThreads / G, KP = G * V;
-  using Vec = VecT<T, V>;
-  __shared__ T sp[SB * KP];
-  __shared__ T sq[HEA_MAX_S * SB * KP];
+  constexpr int G = W2 > 0 ? 2 : 1, SB = kGatherThreads / G, KPL = 4 * G;
+  __shared__ T sp[SB * KPL];
+  __shared__ T sq[HEA_MAX_S * SB * KPL];
   __shared__ double red[4 * HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
   const HeaConst& P = *pc;
   const int N = P.N, K = P.K, S = P.S;
   const int nb = gridDim.x, blk = xcd_block(blockIdx.x, nb);
-  const int tl = threadIdx.x, il = tl / G, b = tl % G;
+  const int tl = threadIdx.x, b = tl / SB, il = tl % SB;  // b is uniform per wave
   const int i = blk * SB + il;
-  T* myp = sp + il * KP + b * V;
   if (i < N) {
-    const Vec own = *reinterpret_cast<const Vec*>(pp + (size_t)i * KP + b * V);
-#pragma unroll
-    for (int v = 0; v < V; ++v) myp[v] = own.x[v];
-    for (int s = 0; s < S; ++s) {
-      const int32_t* col = nbrT + (size_t)P.off[s] * N + i;
-      const T* row = pp + b * V;
-      const int Z = P.Z[s];
-      double acc[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) acc[v] = 0.0;
-      int k = 0;
-      for (; k + 4 <= Z; k += 4) {
-        const int j0 = col[(size_t)k * N], j1 = col[(size_t)(k + 1) * N];
-        const int j2 = col[(size_t)(k + 2) * N], j3 = col[(size_t)(k + 3) * N];
-        const Vec v0 = *reinterpret_cast<const Vec*>(row + (size_t)j0 * KP);
-        const Vec v1 = *reinterpret_cast<const Vec*>(row + (size_t)j1 * KP);
-        const Vec v2 = *reinterpret_cast<const Vec*>(row + (size_t)j2 * KP);
-        const Vec v3 = *reinterpret_cast<const Vec*>(row + (size_t)j3 * KP);
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          acc[v] += (double)v0.x[v];
-          acc[v] += (double)v1.x[v];
-          acc[v] += (double)v2.x[v];
-          acc[v] += (double)v3.x[v];
-        }
-      }
-      for (; k < Z; ++k) {
-        const Vec vk = *reinterpret_cast<const Vec*>(row + (size_t)col[(size_t)k * N] * KP);
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] += (double)vk.x[v];
-      }
-      Vec qv;
-#pragma unroll
-      for (int v = 0; v < V; ++v) qv.x[v] = (T)acc[v];
-      T* myq = sq + s * SB * KP + il * KP + b * V;
-#pragma unroll
-      for (int v = 0; v < V; ++v) myq[v] = qv.x[v];
-      *reinterpret_cast<Vec*>(Q + ((size_t)s * N + i) * KP + b * V) = qv;
-    }
-  } else {
-#pragma unroll
-    for (int v = 0; v < V; ++v) myp[v] = (T)0;
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-      for (int v = 0; v < V; ++v) sq[s * SB * KP + il * KP + b * V + v] = (T)0;
+    if (b == 0)
+      gather_part<T, 4, KPL>(P, nbrT, pp, Q, i, SB, il, 0, sp, sq);
+    else if constexpr (W2 > 0)
+      gather_part<T, W2, KPL>(P, nbrT, pp + (size_t)4 * N, Q + (size_t)4 * S * N, i, SB, il, 4,
+                              sp, sq);
+  }
+  // zeros for absent sites and the padding columns the pair sums never read
+  // beyond K; every LDS column of a present site is written above or here
+  const int c0 = b * 4, cw = b ? KPL - 4 : 4, cset = b ? W2 : 4;
+  for (int v = (i < N ? cset : 0); v < cw; ++v) {
+    sp[il * KPL + c0 + v] = (T)0;
+    for (int s = 0; s < S; ++s) sq[s * SB * KPL + il * KPL + c0 + v] = (T)0;
   }
   __syncthreads();
   const int E = S * K * K;
   for (int t = tl; t < 4 * E; t += kGatherThreads) {
     const int e = t % E, r = t / E;
     const int s = e / (K * K), a = (e / K) % K, c = e % K;
-    const T* q = sq + s * SB * KP;
+    const T* q = sq + s * SB * KPL;
     double acc = 0.0;
-    for (int x = r; x < SB; x += 4) acc += (double)sp[x * KP + a] * (double)q[x * KP + c];
+    for (int x = r; x < SB; x += 4) acc += (double)sp[x * KPL + a] * (double)q[x * KPL + c];
     red[r * E + e] = acc;
   }
   __syncthreads();
@@ -194,39 +222,56 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const double* __restrict__ 
 }
 
 // grad[k][c] = p[k][c] (g[k][c] - sum_d p[k][d] g[k][d]),
-// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; KP / 4 lanes
-// per site, four columns c each.  Every lane loads the site's whole padded
-// Q rows and its own four p columns as vectors before the workgroup builds
-// the symmetrised table H in LDS, so the loads overlap that step; the two
-// halves of <p, g> meet through one shuffle.  grad is [N][K].
-template <typename T, int KP>
+// g[k][c] = sum_s sum_b (dn[s][c][b] + dn[s][b][c]) Q[s][k][b]; one lane
+// per site for K <= 4, else two, four columns c each.  Every lane loads the
+// site's whole Q rows and p row (both parts, as vectors) before the
+// workgroup builds the symmetrised table H in LDS, so the loads overlap that
+// step; the two halves of <p, g> meet through one shuffle.  grad is [N][K].
+template <typename T, int W2>
 __global__ __launch_bounds__(256) void k_adjoint(const HeaConst* __restrict__ pc,
                                                  const T* __restrict__ p,
                                                  const T* __restrict__ Q,
                                                  const double* __restrict__ dn,
                                                  T* __restrict__ grad) {
-  constexpr int GL = KP / 4;
-  using Row = VecT<T, KP>;
-  using Vec = VecT<T, 4>;
+  constexpr int GL = W2 > 0 ? 2 : 1, C = 4 + W2;
+  using VA = VecT<T, 4>;
+  using VB = VecT<T, (W2 > 0 ? W2 : 1)>;
   const HeaConst& P = *pc;
   const int K = P.K, S = P.S, N = P.N;
   __shared__ double H[HEA_MAX_S * HEA_MAX_K * HEA_MAX_K];
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int k = t / GL, b = t % GL;
   const bool live = k < N;
-  Row q[HEA_MAX_S];
-  Vec pr;
+  T q[HEA_MAX_S][C], pr[C];
+  if (live) {
 #pragma unroll
-  for (int s = 0; s < HEA_MAX_S; ++s)
-    if (live && s < S) q[s] = *reinterpret_cast<const Row*>(Q + ((size_t)s * N + k) * KP);
-  if (live) pr = *reinterpret_cast<const Vec*>(p + (size_t)k * KP + b * 4);
+    for (int s = 0; s < HEA_MAX_S; ++s)
+      if (s < S) {
+        const VA qa = *reinterpret_cast<const VA*>(Q + ((size_t)s * N + k) * 4);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) q[s][v] = qa.x[v];
+        if constexpr (W2 > 0) {
+          const VB qb = *reinterpret_cast<const VB*>(Q + (size_t)4 * S * N + ((size_t)s * N + k) * W2);
+#pragma unroll
+          for (int v = 0; v < W2; ++v) q[s][4 + v] = qb.x[v];
+        }
+      }
+    const VA pa = *reinterpret_cast<const VA*>(p + (size_t)k * 4);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pr[v] = pa.x[v];
+    if constexpr (W2 > 0) {
+      const VB pb = *reinterpret_cast<const VB*>(p + (size_t)4 * N + (size_t)k * W2);
+#pragma unroll
+      for (int v = 0; v < W2; ++v) pr[4 + v] = pb.x[v];
+    }
+  }
   for (int e = threadIdx.x; e < S * K * K; e += blockDim.x) {
     const int s = e / (K * K), c = (e / K) % K, bb = e % K;
     H[e] = dn[(s * K + c) * K + bb] + dn[(s * K + bb) * K + c];
   }
   __syncthreads();
   if (!live) return;
-  double g[4], dot = 0.0;
+  double g[4], pk[4], dot = 0.0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int c = b * 4 + v;
@@ -234,16 +279,20 @@ __global__ __launch_bounds__(256) void k_adjoint(const HeaConst* __restrict__ pc
 #pragma unroll
     for (int s = 0; s < HEA_MAX_S; ++s)
 #pragma unroll
-      for (int bb = 0; bb < KP; ++bb)
-        if (c < K && s < S && bb < K) a += H[(s * K + c) * K + bb] * (double)q[s].x[bb];
+      for (int bb = 0; bb < C; ++bb)
+        if (c < K && s < S && bb < K) a += H[(s * K + c) * K + bb] * (double)q[s][bb];
     g[v] = a;
-    if (c < K) dot += (double)pr.x[v] * a;
+    pk[v] = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < C; ++cc)  // this lane's column c of the p row
+      if (cc == c) pk[v] = (double)pr[cc];
+    if (c < K) dot += pk[v] * a;
   }
   if (GL > 1) dot += __shfl_xor(dot, 1, GL);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int c = b * 4 + v;
-    if (c < K) grad[(size_t)k * K + c] = (T)((double)pr.x[v] * (g[v] - dot));
+    if (c < K) grad[(size_t)k * K + c] = (T)(pk[v] * (g[v] - dot));
   }
 }
 
@@ -276,39 +325,75 @@ __global__ void k_site_gain(const HeaConst* __restrict__ pc, const int32_t* __re
   for (int b = 0; b < K; ++b) out[b] = (float)(g[b] - g[a]);
 }
 
-// Columns of the padded occupancy rows.
-inline int relax_cols(int K) { return K <= 4 ? 4 : 8; }
+// Elements of one site's p row (both parts; the file header's layout).
+inline int relax_cols(int K) { return 4 + relax_w2(K); }
 
-// Workgroups of the gather (sites per workgroup: kGatherThreads / (KP / 4)).
+// Workgroups of the gather (sites per workgroup: 512 for K <= 4, else 256).
 inline int relax_gather_blocks(int N, int K) {
-  const int sb = kGatherThreads / (relax_cols(K) / kGatherVec);
+  const int sb = kGatherThreads / (relax_w2(K) > 0 ? 2 : 1);
   return (N + sb - 1) / sb;
 }
 
-template <typename T, int KP>
+template <typename T, int W2>
 hipError_t relax_forward_g(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                            const void* th, void* p, void* Q, double* part, double* n,
                            hipStream_t st) {
-  constexpr int SBS = kSoftmaxThreads / (KP / 4);
-  hipLaunchKernelGGL((k_softmax<T, KP>), dim3((hc.N + SBS - 1) / SBS), dim3(kSoftmaxThreads), 0,
-                     st, static_cast<const T*>(th), static_cast<T*>(p), hc.N, hc.K);
+  hipLaunchKernelGGL((k_softmax<T, W2>), dim3((hc.N + kSoftmaxThreads - 1) / kSoftmaxThreads),
+                     dim3(kSoftmaxThreads), 0, st, static_cast<const T*>(th), static_cast<T*>(p),
+                     hc.N, hc.K);
   const int nb = relax_gather_blocks(hc.N, hc.K);
-  hipLaunchKernelGGL((k_gather_pairs<T, KP / kGatherVec, kGatherVec>), dim3(nb),
-                     dim3(kGatherThreads), 0, st, dc, nbrT, static_cast<const T*>(p),
-                     static_cast<T*>(Q), part);
+  hipLaunchKernelGGL((k_gather_pairs<T, W2>), dim3(nb), dim3(kGatherThreads), 0, st, dc, nbrT,
+                     static_cast<const T*>(p), static_cast<T*>(Q), part);
   hipLaunchKernelGGL(k_pair_reduce, dim3(hc.S * hc.K * hc.K), dim3(256), 0, st, part, nb, n);
   return hipGetLastError();
 }
 
-template <typename T, int KP>
+template <typename T, int W2>
 hipError_t relax_backward_t(const HeaConst* dc, const HeaConst& hc, const void* p, const void* Q,
                             const double* dn, void* grad, hipStream_t st) {
-  const int lanes = hc.N * (KP / 4);
-  hipLaunchKernelGGL((k_adjoint<T, KP>), dim3((lanes + 255) / 256), dim3(256), 0, st, dc,
+  const int lanes = hc.N * (W2 > 0 ? 2 : 1);
+  hipLaunchKernelGGL((k_adjoint<T, W2>), dim3((lanes + 255) / 256), dim3(256), 0, st, dc,
                      static_cast<const T*>(p), static_cast<const T*>(Q), dn,
                      static_cast<T*>(grad));
   return hipGetLastError();
 }
+
+// Calls F<T, W2>() for the problem's species count and dtype.
+template <template <typename, int> class F, typename... Args>
+hipError_t relax_dispatch(int dtype, int K, Args... args) {
+  const int w2 = relax_w2(K);
+  if (dtype == 1) {
+    switch (w2) {
+      case 0: return F<double, 0>::run(args...);
+      case 1: return F<double, 1>::run(args...);
+      case 2: return F<double, 2>::run(args...);
+      default: return F<double, 4>::run(args...);
+    }
+  }
+  switch (w2) {
+    case 0: return F<float, 0>::run(args...);
+    case 1: return F<float, 1>::run(args...);
+    case 2: return F<float, 2>::run(args...);
+    default: return F<float, 4>::run(args...);
+  }
+}
+
+template <typename T, int W2>
+struct RelaxForward {
+  static hipError_t run(const HeaConst* dc, const HeaConst* hc, const int32_t* nbrT,
+                        const void* th, void* p, void* Q, double* part, double* n,
+                        hipStream_t st) {
+    return relax_forward_g<T, W2>(dc, *hc, nbrT, th, p, Q, part, n, st);
+  }
+};
+
+template <typename T, int W2>
+struct RelaxBackward {
+  static hipError_t run(const HeaConst* dc, const HeaConst* hc, const void* p, const void* Q,
+                        const double* dn, void* grad, hipStream_t st) {
+    return relax_backward_t<T, W2>(dc, *hc, p, Q, dn, grad, st);
+  }
+};
 
 }  // namespace
 
@@ -331,23 +416,13 @@ hipError_t hea_relax_transpose_nbr(const int32_t* nbr, int N, int stride, int32_
 hipError_t hea_relax_forward_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,
                                   int dtype, const void* th, void* p, void* Q, double* part,
                                   double* n, hipStream_t st) {
-  const bool g4 = relax_cols(hc.K) == 4;
-  if (dtype == 1)
-    return g4 ? relax_forward_g<double, 4>(dc, hc, nbrT, th, p, Q, part, n, st)
-              : relax_forward_g<double, 8>(dc, hc, nbrT, th, p, Q, part, n, st);
-  return g4 ? relax_forward_g<float, 4>(dc, hc, nbrT, th, p, Q, part, n, st)
-            : relax_forward_g<float, 8>(dc, hc, nbrT, th, p, Q, part, n, st);
+  return relax_dispatch<RelaxForward>(dtype, hc.K, dc, &hc, nbrT, th, p, Q, part, n, st);
 }
 
 hipError_t hea_relax_backward_impl(const HeaConst* dc, const HeaConst& hc, int dtype,
                                    const void* p, const void* Q, const double* dn, void* grad,
                                    hipStream_t st) {
-  const bool g4 = relax_cols(hc.K) == 4;
-  if (dtype == 1)
-    return g4 ? relax_backward_t<double, 4>(dc, hc, p, Q, dn, grad, st)
-              : relax_backward_t<double, 8>(dc, hc, p, Q, dn, grad, st);
-  return g4 ? relax_backward_t<float, 4>(dc, hc, p, Q, dn, grad, st)
-            : relax_backward_t<float, 8>(dc, hc, p, Q, dn, grad, st);
+  return relax_dispatch<RelaxBackward>(dtype, hc.K, dc, &hc, p, Q, dn, grad, st);
 }
 
 hipError_t hea_site_gain_impl(const HeaConst* dc, const HeaConst& hc, const int32_t* nbrT,

@@ -4,8 +4,8 @@
 (float64) with the HIP forward (softmax, neighbour gather, deterministic
 FP64 reduction) and the hand-written HIP backward (neighbour-gather adjoint)
 of ``hea_relax_forward`` / ``hea_relax_backward``; the occupancies and
-neighbour sums it keeps for the backward have rows padded to
-``relax_cols(K)`` columns.  ``soft_fitness_sq`` builds the relaxed
+neighbour sums it keeps for the backward hold ``relax_cols(K)`` elements
+per site (the library's row layout; 5 for five species).  ``soft_fitness_sq`` builds the relaxed
 objective on top with ordinary torch ops:
 
     gamma = n / (Z_s N),  alpha = 1 - gamma / (c_a c_b),
@@ -34,9 +34,9 @@ def _dtype_code(t: torch.Tensor) -> int:
 
 
 def relax_cols(K: int) -> int:
-    """Columns of the padded occupancy rows p [N][KP] and neighbour sums
-    q [S][N][KP] (zeros beyond K), as the library sizes them
-    (hea_relax_cols)."""
+    """Elements per site of the occupancies p [N * KP] and neighbour sums
+    q [S * N * KP] that the forward keeps for the backward, as the library
+    lays them out (hea_relax_cols)."""
     kp = int(Nat.lib().hea_relax_cols(int(K)))
     if kp < 0:
         raise ValueError(f"no relaxation layout for {K} species")

@@ -117,11 +117,12 @@ def test_python_wrapper_raises_value_error(spec):
 
 
 def test_relaxation_row_width_comes_from_the_library():
-    """p and q rows are padded to 4 columns up to 4 species, else 8; the
-    library's hea_relax_cols is the one source of that rule (no GPU needed)."""
+    """p and q hold 4 elements per site up to 4 species, then 4 + 1, 4 + 2
+    and 4 + 4 (columns 0-3, then the rest); the library's hea_relax_cols is
+    the one source of that rule (no GPU needed)."""
     from pyhea_amd.relax import relax_cols
 
-    assert [relax_cols(k) for k in range(1, 9)] == [4, 4, 4, 4, 8, 8, 8, 8]
+    assert [relax_cols(k) for k in range(1, 9)] == [4, 4, 4, 4, 5, 6, 8, 8]
     assert N.lib().hea_relax_cols(0) == -1 and N.lib().hea_relax_cols(9) == -1
     with pytest.raises(ValueError):
         relax_cols(9)
