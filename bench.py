@@ -379,7 +379,6 @@ def batched(a):
     total = sum(evals)
     byts = sum(e * bytes_per_eval(sum(sp.shell_sizes[:sp.S])) for e, sp in zip(evals, specs))
     b.close()
-    calib_copy(a.calib_copy_mb)
     out = {
         "metric": "swap evals/s", "value": total / dt, "unit": "swap evals/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps,
