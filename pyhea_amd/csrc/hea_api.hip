@@ -351,6 +351,7 @@ struct hea_problem_s {
   int device = 0;
   HeaConst hc;
   bool force_hbm_species = false;  // PYHEA_AMD_SPECIES_IN_HBM=1 (tests)
+  bool walk_gather = false;        // PYHEA_AMD_WALK_GATHER=1: no neighbour records in the walk
   int64_t max_lds = 0;
   DevBuf<HeaConst> d_const;
   DevBuf<int32_t> d_nbr;
@@ -361,6 +362,7 @@ struct hea_problem_s {
   DevBuf<int64_t> d_stats;
   DevBuf<uint32_t> d_iter;
   DevBuf<uint32_t> d_tie;  // tie flag of the population sort (zero between calls)
+  DevBuf<unsigned long long> d_rec;  // neighbour records of the walk (species in HBM)
   // relaxation: per-workgroup pair-count partials and the slot-major copy
   // of the neighbour table
   DevBuf<double> d_relax_part;
@@ -385,8 +387,9 @@ struct hea_session_s {
   DevBuf<uint32_t> tie;            // tie flag of the population sorts (zero between sorts)
   DevBuf<unsigned long long> clk;  // [0] start, [1] first fitness <= thr
   int iters_done = 0;
-  DevBuf<unsigned long long> hist;  // guided mode, [L][N]
+  DevBuf<unsigned long long> hist;  // neighbour records [L][N]: guided mode and/or the walk
   GuideCfg guide;
+  unsigned long long* rec = nullptr;  // records of the walk (species in HBM), or nullptr
   int tp = 0;             // proposals per pass (0: default)
 };
 
@@ -534,7 +537,8 @@ void fill_problem_args(hea_problem p, HeaMcArgs& a) {
 int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, int64_t* stats,
               const uint32_t* iter_ptr, unsigned long long* first_hit, int L, int task,
               int depth, uint64_t seed, uint32_t phase, double thr, size_t lds, bool sp_lds,
-              hipStream_t st, const GuideCfg& gd = GuideCfg(), int tp = 0) {
+              hipStream_t st, const GuideCfg& gd = GuideCfg(), int tp = 0,
+              unsigned long long* rec = nullptr) {
   HeaMcArgs a;
   a.pc = p->d_const.p;
   a.nbr = p->d_nbr.p;
@@ -557,6 +561,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.g_multi = gd.multi;
   a.dbg_pairs = gd.dbg_pairs;
   a.dbg_pred = gd.dbg_pred;
+  a.rec = rec;
   const HeaConst& c = p->hc;
   a.N = c.N;
   a.K = c.K;
@@ -568,7 +573,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
     a.delta[s] = c.delta[s];
     a.weight[s] = c.weight[s];
   }
-  hea_mc_split(a, p->hc.self_nbr != 0);
+  a.dw = hea_mc_draw_lanes(task, c.conc, c.K, hea_mc_split(a, p->hc.self_nbr != 0));
   HEA_HIP(hea_launch_local_mc(a, p->hc.K, sp_lds, p->hc.self_nbr != 0, lds, st));
   return HEA_OK;
 }
@@ -580,6 +585,13 @@ int check_run_args(hea_problem p, int L, int task, int depth) {
     return fail(HEA_EINVAL, "parallel_task must be in [1, 2048]");
   if (depth < 0) return fail(HEA_EINVAL, "converge_depth must be >= 0");
   return HEA_OK;
+}
+
+// Whether the walk of a launch with this species placement uses neighbour
+// records: species read from HBM, records that fit one word, no override.
+bool use_records(hea_problem p, bool sp_lds) {
+  return !sp_lds && !p->walk_gather &&
+         hea_records_fit(p->hc.K, p->hc.S, p->hc.Z, p->hc.self_nbr != 0);
 }
 
 // LDS bytes of a local-search launch; species live in LDS when they fit.
@@ -666,6 +678,8 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
   {
     const char* v = getenv("PYHEA_AMD_SPECIES_IN_HBM");
     p->force_hbm_species = v && v[0] == '1';
+    const char* w = getenv("PYHEA_AMD_WALK_GATHER");
+    p->walk_gather = w && w[0] == '1';
   }
   HeaConst& c = p->hc;
   memset(&c, 0, sizeof(c));
@@ -794,20 +808,28 @@ int hea_generate(hea_problem p, int32_t L, uint64_t seed, uint32_t iter, uint32_
 
 // Device work of one local search (no sort): iteration word, zeroed
 // counters, pair-count recount, search.  iter_w [1], stats [4L] and counts
-// [L][S][K][K] are the caller's workspace; nothing is allocated here.
+// [L][S][K][K] are the caller's workspace.  With `records` the walk of
+// lattices whose species stay in HBM uses neighbour records kept in the
+// problem's scratch (allocated here); without, nothing is allocated.
 static int local_mc_enqueue(hea_problem p, int L, uint8_t* lat, double* fit, int task, int depth,
                             uint64_t seed, uint32_t iter, uint32_t phase, uint32_t* iter_w,
-                            int64_t* stats, int32_t* counts, hipStream_t st) {
+                            int64_t* stats, int32_t* counts, hipStream_t st, bool records) {
   size_t lds;
   bool sp_lds;
   int rc = mc_lds(p, task, &lds, &sp_lds);
   if (rc) return rc;
+  unsigned long long* rec = nullptr;
+  if (records && use_records(p, sp_lds)) {
+    const size_t n = (size_t)L * p->hc.N;
+    if (p->d_rec.n < n) HEA_HIP(p->d_rec.alloc(n));
+    rec = p->d_rec.p;
+  }
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, iter_w, iter);
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, st, reinterpret_cast<int32_t*>(stats),
                      (size_t)8 * L);
   if ((rc = launch_recount(p, lat, counts, L, st))) return rc;
   return launch_mc(p, lat, fit, counts, stats, iter_w, nullptr, L, task, depth, seed, phase, -1.0,
-                   lds, sp_lds, st);
+                   lds, sp_lds, st, GuideCfg(), 0, rec);
 }
 
 static int local_mc_dev(hea_problem p, int L, uint8_t* lat, double* fit, int task, int depth,
@@ -815,7 +837,7 @@ static int local_mc_dev(hea_problem p, int L, uint8_t* lat, double* fit, int tas
   int rc = ensure_scratch(p, L);
   if (rc) return rc;
   if ((rc = local_mc_enqueue(p, L, lat, fit, task, depth, seed, iter, phase, p->d_iter.p,
-                             p->d_stats.p, p->d_counts.p, st)))
+                             p->d_stats.p, p->d_counts.p, st, true)))
     return rc;
   if (sort) {
     if ((rc = launch_merge(lat, fit, L, lat, fit, 0, L, p->d_perm.p, p->d_tie.p, p->d_lat2.p,
@@ -888,10 +910,24 @@ int hea_local_mc_device_noalloc(hea_problem p, int32_t L, void* d_lattices, void
                           static_cast<double*>(d_fitness), task, depth, seed, iter, phase,
                           reinterpret_cast<uint32_t*>(ws), reinterpret_cast<int64_t*>(ws + 256),
                           reinterpret_cast<int32_t*>(ws + 256 + stats_b),
-                          static_cast<hipStream_t>(stream));
+                          static_cast<hipStream_t>(stream), false);
 }
 
 // ------------------------------------------------------------------ session --
+
+// Neighbour records of a session's walk: used when its species stay in HBM
+// (the buffer is guided mode's records when guided, the same size).
+static int session_records(hea_session s) {
+  s->rec = nullptr;
+  if (!use_records(s->p, s->sp_lds)) return HEA_OK;
+  const size_t n = (size_t)s->L * s->p->hc.N;
+  if (s->hist.n < n) {
+    HEA_HIP(s->hist.alloc(n));
+    if (s->guide.hist) s->guide.hist = s->hist.p;
+  }
+  s->rec = s->hist.p;
+  return HEA_OK;
+}
 
 // Optional measurement hooks of one iteration: events around both local
 // searches and stats snapshots before/after them (eager runs only).
@@ -919,7 +955,7 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
   }
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
                       s->task, s->depth, s->seed, HEA_PHASE_LS1, s->thr, s->lds, s->sp_lds, st,
-                      s->guide, s->tp)))
+                      s->guide, s->tp, s->rec)))
     return rc;
   if (hk) {
     HEA_HIP(hipEventRecord(hk->ev[1], st));
@@ -937,7 +973,7 @@ static int session_enqueue_iteration(hea_session s, const IterHooks* hk = nullpt
   if (hk) HEA_HIP(hipEventRecord(hk->ev[2], st));
   if ((rc = launch_mc(p, s->B.p, s->fB.p, s->counts.p, s->stats.p, s->iter.p, s->clk.p + 1, L,
                       s->task, s->depth, s->seed, HEA_PHASE_LS2, s->thr, s->lds, s->sp_lds, st,
-                      s->guide, s->tp)))
+                      s->guide, s->tp, s->rec)))
     return rc;
   if (hk) {
     HEA_HIP(hipEventRecord(hk->ev[3], st));
@@ -985,7 +1021,7 @@ int hea_session_create(hea_problem p, int32_t L, int32_t task, int32_t depth, do
     delete s;
     return fail(HEA_ENOMEM, std::string("session allocation: ") + hipGetErrorString(e));
   }
-  if ((rc = mc_lds(p, task, &s->lds, &s->sp_lds))) {
+  if ((rc = mc_lds(p, task, &s->lds, &s->sp_lds)) || (rc = session_records(s))) {
     delete s;
     return rc;
   }
@@ -1013,6 +1049,7 @@ int hea_session_tile(hea_session s, int32_t tp) {
   s->tp = tp;
   s->lds = lds;
   s->sp_lds = sp;
+  if ((rc = session_records(s))) return rc;
   if (s->exec) {
     (void)hipGraphExecDestroy(s->exec);
     (void)hipGraphDestroy(s->graph);
@@ -1090,6 +1127,7 @@ int hea_session_guide(hea_session s, int32_t candidates, double tau, int32_t acc
   s->guide.cand = candidates;
   s->guide.tau = tau;
   s->guide.multi = accepts;
+  if ((rc = session_records(s))) return rc;
   if (s->exec) {  // the graph bakes the launch arguments in: capture again
     (void)hipGraphExecDestroy(s->exec);
     (void)hipGraphDestroy(s->graph);
@@ -1273,6 +1311,7 @@ struct hea_batch_s {
 };
 
 extern "C" int hea_batch_grid(hea_batch b, int32_t grid);  // autotuner hook (below)
+static int batch_records(hea_batch b);
 
 static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
   const int L = b->L, P = (int)b->probs.size();
@@ -1447,7 +1486,9 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
       a.task = task;
       a.depth = depth;
       a.tp = b->tp;
-      b->threads = std::max(b->threads, hea_mc_split(a, false));
+      const int lanes = hea_mc_split(a, false);
+      a.dw = hea_mc_draw_lanes(task, probs[q]->hc.conc, probs[q]->hc.K, lanes);
+      b->threads = std::max(b->threads, lanes);
       a.thr = thr;
       a.seed = seed + (uint64_t)q;
       a.phase = (uint32_t)(HEA_PHASE_LS1 + ph);
@@ -1464,6 +1505,10 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   int cus = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   b->grid = std::min((int)items.size(), std::max(1, cus) * 2);
+  if ((rc = batch_records(b))) {
+    delete b;
+    return rc;
+  }
   *out = b;
   return HEA_OK;
 }
@@ -1490,6 +1535,28 @@ static int batch_patch_args(hea_batch b, const std::function<void(HeaMcArgs&, in
   return HEA_OK;
 }
 
+// Neighbour records of the batch's walks: problems whose species stay in HBM
+// use their slice of b->hist (guided mode's records when guided).
+static int batch_records(hea_batch b) {
+  const int P = (int)b->probs.size();
+  std::vector<char> use(P);
+  std::vector<size_t> off(P);
+  size_t n = 0;
+  bool any = false;
+  for (int q = 0; q < P; ++q) {
+    use[q] = use_records(b->probs[q], b->sp_lds);
+    any = any || use[q];
+    off[q] = n;
+    n += (size_t)b->L * b->probs[q]->hc.N;
+  }
+  if (any && b->hist.n < n) {
+    if (b->guide.hist) return fail(HEA_EINVAL, "guided records buffer too small");
+    HEA_HIP(b->hist.alloc(n));
+  }
+  unsigned long long* base = b->hist.p;
+  return batch_patch_args(b, [&](HeaMcArgs& a, int q) { a.rec = use[q] ? base + off[q] : nullptr; });
+}
+
 int hea_batch_tile(hea_batch b, int32_t tp) {
   if (!b || !valid_tp(tp, b->S))
     return fail(HEA_EINVAL, "tile must be 64, 128, 192 or 256");
@@ -1505,10 +1572,13 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   }
   HEA_HIP(hipSetDevice(b->probs[0]->device));
   HEA_HIP(hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds));
-  int rc = batch_patch_args(b, [tp, b](HeaMcArgs& a, int) {
+  int rc = batch_patch_args(b, [tp, b](HeaMcArgs& a, int q) {
     a.tp = tp;
-    b->threads = std::max(b->threads, hea_mc_split(a, false));
+    const int lanes = hea_mc_split(a, false);
+    a.dw = hea_mc_draw_lanes(b->task, b->probs[q]->hc.conc, b->probs[q]->hc.K, lanes);
+    b->threads = std::max(b->threads, lanes);
   });
+  if (!rc) rc = batch_records(b);
   if (rc) return rc;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
@@ -1542,6 +1612,7 @@ int hea_batch_guide(hea_batch b, int32_t candidates, double tau, int32_t accepts
   });
   if (rc) return rc;
   b->guide.hist = b->hist.p;
+  if ((rc = batch_records(b))) return rc;
   b->guide.cand = candidates;
   b->guide.tau = tau;
   b->guide.multi = accepts;

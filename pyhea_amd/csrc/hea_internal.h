@@ -51,6 +51,7 @@ struct HeaMcArgs {
   unsigned long long* first_hit;  // wall clock of the first fitness <= thr (atomicMin)
   uint64_t seed;
   int32_t L, task, depth, tp;   // tp: proposal slots per pass
+  int32_t dw;                   // draw lanes per pass with species in HBM (0: every lane)
   // screen lanes (hea_mc_split): shell s's group starts at lane gbase[s] and
   // walks its list in 2^glog[s] parts; gbase[S] = lanes of the workgroup
   int32_t gbase[HEA_MAX_S + 1];
@@ -68,6 +69,10 @@ struct HeaMcArgs {
   // then returns without searching
   int32_t* dbg_pairs;
   float* dbg_pred;
+  // neighbour records for the screen walk when species are read from HBM
+  // (nullptr: the walk gathers every neighbour's species).  Guided mode
+  // keeps its records in hist and sets rec = hist to use them.
+  unsigned long long* rec;      // [L][N] (HeaHistLayout)
   // problem scalars, copied from HeaConst so they live in the kernarg segment
   int32_t N, K, S, stride;
   int32_t Z[HEA_MAX_S], off[HEA_MAX_S];
@@ -89,11 +94,15 @@ struct HeaMcBatchArgs {
   uint32_t* queue;  // [HEA_XCDS] next item of each part, zeroed before each launch
 };
 
-// Guided mode's per-site record: the species counts of a site's neighbours,
-// all shells in one 64-bit word.  The count of species x < K-1 in shell s is
+// Per-site neighbour record: the species counts of a site's neighbours, all
+// shells in one 64-bit word.  The count of species x < K-1 in shell s is
 // the field at bit x*W + o[s] of width w[s] (the bit width of Z_s; W = sum of
 // the widths); species K-1's count is Z_s minus the others.  Usable when
 // (K-1)*W <= 64 (FCC 12/6/24 needs 12 bits per species, BCC 8/6/12 11).
+// Guided mode scores candidates from them; when species are read from HBM
+// the screen walk takes its species histograms from them as well (exact:
+// the neighbour relation is symmetric, checked when a problem is created,
+// so a swap changes exactly the records of the two sites' neighbours).
 struct HeaHistLayout {
   int w[HEA_MAX_S], o[HEA_MAX_S], W;
 };
@@ -110,6 +119,12 @@ __host__ __device__ inline HeaHistLayout hea_hist_layout(int S, const int32_t* Z
   return h;
 }
 
+// True when a problem's records fit one 64-bit word and the walk may use
+// them (cells whose sites list themselves keep the gathering walk).
+inline bool hea_records_fit(int K, int S, const int32_t* Z, bool self) {
+  return !self && K <= 6 && (K - 1) * hea_hist_layout(S, Z).W <= 64;
+}
+
 // R_s(k) from the binade segments (exact: the host verified every k).
 __device__ __forceinline__ double hea_r_of_count(const HeaConst& P, int s, int64_t k) {
   int q = 0;
@@ -122,6 +137,7 @@ size_t hea_mc_lds_bytes(int kmax, int task, int tp, int n_sites, bool sp_lds);
 int hea_mc_kmax(int K, bool self);
 int hea_mc_tp(int task, int S);
 int hea_mc_split(HeaMcArgs& a, bool self);
+int hea_mc_draw_lanes(int task, const double* conc, int K, int lanes);
 hipError_t hea_prepare_local_mc(int K, bool sp_lds, bool self, size_t lds);
 hipError_t hea_launch_local_mc(const HeaMcArgs& a, int K, bool sp_lds, bool self, size_t lds,
                                hipStream_t st);

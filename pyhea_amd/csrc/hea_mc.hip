@@ -53,6 +53,7 @@
 // (-ffp-contract=off); the screen uses explicit fma where its error bound
 // allows.
 
+#include <cmath>
 #include <type_traits>
 
 #include "hea_internal.h"
@@ -171,6 +172,57 @@ __device__ __forceinline__ void walk_pair(const int32_t* __restrict__ r1,
   }
 }
 
+// Occurrences of site a2 among the z entries of list r1 (NC 16-byte chunks,
+// all loads in flight together); entries at or beyond z are masked for a
+// part of a split list (a whole list ends in its padding, -1).
+template <bool PART, int NC>
+__device__ __forceinline__ int count_site(const int32_t* __restrict__ r1, int z, int a2) {
+  int4 u[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) u[c] = reinterpret_cast<const int4*>(r1)[c];
+  int n = 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int nu[4] = {u[c].x, u[c].y, u[c].z, u[c].w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) n += ((!PART || 4 * c + t < z) && nu[t] == a2) ? 1 : 0;
+  }
+  return n;
+}
+
+template <bool PART>
+__device__ __forceinline__ int count_site(const int32_t* __restrict__ r1, int z, int nch, int a2) {
+  switch (nch) {  // uniform across the wave
+#define HEA_COUNT(n) \
+  case n: return count_site<PART, n>(r1, z, a2);
+    HEA_COUNT(1) HEA_COUNT(2) HEA_COUNT(3) HEA_COUNT(4) HEA_COUNT(5) HEA_COUNT(6) HEA_COUNT(7)
+    HEA_COUNT(8)
+#undef HEA_COUNT
+    default: return 0;
+  }
+}
+
+// Shell-s species histogram of a neighbour list in the walk's packing, from
+// the list owner's neighbour record r (HeaHistLayout: the shell's fields
+// start at bit o, are w bits wide and W bits apart); z = Z_s.
+template <int KMAX>
+__device__ __forceinline__ typename Pack<KMAX>::T rec_pack(unsigned long long r, int o, int w,
+                                                           int W, int K, int z) {
+  using PK = Pack<KMAX>;
+  typename PK::T m = 0;
+  int rest = z;
+  const unsigned long long fm = (1ull << w) - 1ull;
+#pragma unroll
+  for (int x = 0; x < KMAX - 1; ++x) {
+    if (x < K - 1) {
+      const int f = (int)((r >> (x * W + o)) & fm);
+      rest -= f;
+      m += (typename PK::T)f << (PK::W * x);
+    }
+  }
+  return m + ((typename PK::T)rest << (PK::W * (K - 1)));
+}
+
 // LDS carve-up of one workgroup (byte offsets; every region 16-byte aligned).
 template <int KMAX>
 struct LdsLayout {
@@ -203,8 +255,8 @@ struct LdsLayout {
     cl = take(sizeof(int) * (size_t)task);
     sl = take(sizeof(int) * 64);
     sh = take(sizeof(McShared));
-    sp = o;
-    total = sp_lds ? ((o + (size_t)n + 15) & ~(size_t)15) : o;
+    sp = o;  // species [N] when resident, else the proposals' species pairs [task]
+    total = (o + (sp_lds ? (size_t)n : sizeof(uint16_t) * (size_t)task) + 15) & ~(size_t)15;
   }
 };
 
@@ -233,6 +285,7 @@ struct Lds {
   int* sl;       // [64] guided mode: the round's shortlist for further accepts
   McShared* sh;
   uint8_t* sp;   // [N] when resident
+  uint16_t* pab; // [task] species in HBM: a | b << 8 of each proposal, from the draw
   static size_t bytes(int task, int n, bool sp_lds) { return LdsLayout<KMAX>(task, n, sp_lds).total; }
   __device__ void carve(unsigned char* base, int task, bool sp_lds) {
     const LdsLayout<KMAX> L(task, 0, sp_lds);
@@ -256,6 +309,7 @@ struct Lds {
     sl = reinterpret_cast<int*>(base + L.sl);
     sh = reinterpret_cast<McShared*>(base + L.sh);
     sp = sp_lds ? base + L.sp : nullptr;
+    pab = sp_lds ? nullptr : reinterpret_cast<uint16_t*>(base + L.sp);
   }
   __device__ __forceinline__ int idx(int s, int i, int j) const { return (s * KMAX + i) * KMAX + j; }
   __device__ __forceinline__ int hidx(int s, int i, int j) const { return (s * KMAX + i) * 8 + j; }
@@ -572,8 +626,11 @@ __device__ __forceinline__ void exact_stage(const HeaMcArgs& A, const Lds<KMAX>&
   int a = 0, b = 0;
   if (best_fit < prev_fit) {  // accept: write the winner's rows
     const int2 pr = T.prop[bj];
-    a = sp[pr.x];
-    b = sp[pr.y];
+    {
+      const WalkRec r = walk_unpack(T.WC[bj]);  // shell 0's walk holds the species
+      a = r.a;
+      b = r.b;
+    }
     if (wc0 + cpc < nc) {  // a later chunk overwrote the winner's entries
       if (lane < GS) {
         const int w = s * task + bj;
@@ -670,11 +727,13 @@ __device__ __forceinline__ SwapNbrs swap_nbrs_load(const HeaMcArgs& A, int2 pr, 
   return r;
 }
 
-// Guided mode, after swapping a1 = pr.x (species a -> b) and a2 = pr.y
-// (b -> a): every neighbour of the two sites (n, from swap_nbrs_load with the
-// same first and stride) sees the change in its neighbour-count record, and
-// the two sites and their neighbours are marked in the round's conflict map.
-template <int KMAX>
+// After swapping a1 = pr.x (species a -> b) and a2 = pr.y (b -> a): every
+// neighbour of the two sites (n, from swap_nbrs_load with the same first and
+// stride) sees the change in its neighbour-count record (the relation is
+// symmetric, so these are exactly the records that change); with MAP
+// (guided mode) the two sites and their neighbours are also marked in the
+// round's conflict map.
+template <int KMAX, bool MAP = true>
 __device__ __forceinline__ void guided_apply_swap(const HeaMcArgs& A, const Lds<KMAX>& T,
                                                   unsigned long long* hist,
                                                   const HeaHistLayout& hl, int2 pr, int a, int b,
@@ -689,16 +748,19 @@ __device__ __forceinline__ void guided_apply_swap(const HeaMcArgs& A, const Lds<
     int q = t < zs ? t : t - zs, s = 0;
     while (q >= A.Z[s]) q -= A.Z[s++];
     // the last species has no field (its count is implied)
-    const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + hl.o[s]) : 0ull;
-    const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + hl.o[s]) : 0ull;
+    const int os = s == 0 ? hl.o[0] : s == 1 ? hl.o[1] : hl.o[2];  // no variable index
+    const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + os) : 0ull;
+    const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + os) : 0ull;
     const unsigned long long dv = t < zs ? fb - fa : fa - fb;  // modulo 2^64
     // every access to a lattice's records comes from its own workgroup:
     // workgroup-scope atomics, loads and barriers order them
     __hip_atomic_fetch_add(hist + n.nb[u], dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    const uint32_t mb = map_bit(n.nb[u]);
-    atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
+    if constexpr (MAP) {
+      const uint32_t mb = map_bit(n.nb[u]);
+      atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
+    }
   }
-  if (first < 2) {
+  if (MAP && first < 2) {
     const uint32_t mb = map_bit(first == 0 ? pr.x : pr.y);
     atomicOr(&T.cmap[mb >> 5], 1u << (mb & 31u));
   }
@@ -904,8 +966,13 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   // ballot is published by the accept's barrier.
   uint32_t nx1 = 0, nx2 = 0;
   bool nxt = false;        // nx1, nx2 hold the pair at pos + tid
-  bool nvalid = false;     // its validity, when nbal_ok
+  int npk = 0;             // species in HBM: its species a | b << 8 (valid iff they differ)
+  bool nvalid = false;     // species in LDS: its validity (both when nbal_ok)
   bool nbal_ok = false;    // this wave's ballot of the pass is in wcount already
+  // draw lanes per pass: all lanes when species sit in LDS; with species in
+  // HBM only as many as a round needs with a wide margin (each draw costs
+  // two species loads)
+  const int dw = SP_LDS ? nthr : max(64, min(A.dw > 0 ? A.dw : nthr, nthr));
   int2* dq = reinterpret_cast<int2*>(T.cmap);  // [64] (cmap is unused in plain mode)
 
   constexpr bool guided = GUIDED;
@@ -913,18 +980,30 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   // 8 bytes per site, so the records of many lattices stay cache-resident)
   unsigned long long* hist = guided ? A.hist + (size_t)l * N : nullptr;
   const HeaHistLayout hl = hea_hist_layout(S, A.Z);
-  if constexpr (GUIDED) {
+  // species read from HBM: the screen walk takes its species histograms from
+  // the neighbour records (guided mode's own, or this lattice's slice of
+  // A.rec) instead of gathering every neighbour's species
+  const bool use_rec = !SP_LDS && !SELF && A.rec != nullptr;
+  unsigned long long* rec = use_rec ? (GUIDED ? hist : A.rec + (size_t)l * N) : nullptr;
+  // this wave's shell fields in the records (selected without indexing the
+  // layout by a variable, which would move it to scratch memory)
+  const int r_o = s_me == 0 ? hl.o[0] : s_me == 1 ? hl.o[1] : hl.o[2];
+  const int r_w = s_me == 0 ? hl.w[0] : s_me == 1 ? hl.w[1] : hl.w[2];
+  if (GUIDED || use_rec) {
+    unsigned long long* rb = GUIDED ? hist : rec;
     for (int k = tid; k < N; k += nthr) {
       const int32_t* row = A.nbr + (size_t)k * A.stride;
       unsigned long long m = 0ull;
-      for (int s = 0; s < S; ++s)
-        for (int q = 0; q < A.Z[s]; ++q) {
-          const int c = sp[row[A.off[s] + q]];
-          if (c < K - 1) m += 1ull << (c * hl.W + hl.o[s]);
-        }
-      hist[k] = m;
+#pragma unroll
+      for (int s = 0; s < HEA_MAX_S; ++s)
+        if (s < S)
+          for (int q = 0; q < A.Z[s]; ++q) {
+            const int c = sp[row[A.off[s] + q]];
+            if (c < K - 1) m += 1ull << (c * hl.W + hl.o[s]);
+          }
+      rb[k] = m;
     }
-    __syncthreads();  // every site's record before the first candidate reads
+    __syncthreads();  // every site's record before the first read
   }
   // guided mode: candidate stream key of this lattice and phase
   const uint32_t gk = hea_mix32(hea_mix32(hea_mix32((uint32_t)A.seed ^ 0x6a09e667u) ^
@@ -1009,21 +1088,23 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     int found = guided ? task : 0;
     bool first = true;
     while (found < task) {
-      const uint32_t p = pos + tid;
-      uint32_t s1, s2;
+      const uint32_t p = pos + tid;  // lanes tid < dw draw
+      uint32_t s1 = 0, s2 = 0;
       if (first && nxt) {
         s1 = nx1;
         s2 = nx2;
-      } else {
+      } else if (tid < dw) {
         hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, p, (uint32_t)N, &s1, &s2);
       }
       bool valid;
       unsigned long long bal;
+      int ab = npk;  // the pair's species a | b << 8
       if (first && nbal_ok) {
-        valid = nvalid;
+        valid = SP_LDS ? nvalid : (npk & 255) != (npk >> 8);
         bal = __ballot(valid);
       } else {
-        valid = sp[s1] != sp[s2];
+        ab = (tid < dw) ? (int)sp[s1] | ((int)sp[s2] << 8) : 0;
+        valid = (ab & 255) != (ab >> 8);
         bal = __ballot(valid);
         if (lane == 0) T.sh->wcount[wave] = __popcll(bal);
         __syncthreads();
@@ -1043,10 +1124,13 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         }
       }
       const int rank = found + off + pre;
-      if (valid && rank < task) T.prop[rank] = make_int2((int)s1, (int)s2);
+      if (valid && rank < task) {
+        T.prop[rank] = make_int2((int)s1, (int)s2);
+        if constexpr (!SP_LDS) T.pab[rank] = (uint16_t)ab;
+      }
       if (valid && rank == task - 1) T.sh->next_pos = p + 1;
       found += tot;
-      pos += nthr;
+      pos += dw;
       __syncthreads();
       if (found >= task) pos = T.sh->next_pos;
     }
@@ -1076,11 +1160,34 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         PT MA = 0, MB = 0;
         int c11 = 0, c12 = 0, c22 = 0, c21 = 0, a = 0, b = 0;
         if (pr.x >= 0) {
-          a = sp[pr.x];  // before the walk, so these loads overlap the row loads
-          b = sp[pr.y];
+          if constexpr (!SP_LDS && !GUIDED) {  // the draw loaded them
+            const int ab = T.pab[j];
+            a = ab & 255;
+            b = ab >> 8;
+          } else {
+            a = sp[pr.x];  // before the walk, so these loads overlap the row loads
+            b = sp[pr.y];
+          }
           const int32_t* r1 = A.nbr + (size_t)pr.x * A.stride + poff;
           const int32_t* r2 = A.nbr + (size_t)pr.y * A.stride + poff;
-          if (SELF || plg == 0)  // the whole list (lists of cells with self neighbours are never split)
+          if (use_rec) {
+            // histograms from the two records (part 0 of the list only, so
+            // the parts' sums below count them once); from the rows only
+            // the adjacency: c21 = c12 for a symmetric relation, and no
+            // site lists itself
+            unsigned long long ra = 0ull, rb = 0ull;
+            if (part == 0) {
+              ra = __hip_atomic_load(rec + pr.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              rb = __hip_atomic_load(rec + pr.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            c12 = (plg == 0) ? count_site<false>(r1, pz, pch, pr.y)
+                             : count_site<true>(r1, pz, pch, pr.y);
+            c21 = c12;
+            if (part == 0) {
+              MA = rec_pack<KMAX>(ra, r_o, r_w, hl.W, K, A.Z[s_me]);
+              MB = rec_pack<KMAX>(rb, r_o, r_w, hl.W, K, A.Z[s_me]);
+            }
+          } else if (SELF || plg == 0)  // the whole list (lists of cells with self neighbours are never split)
             walk_pair<KMAX, SELF, false>(r1, r2, pz, pch, sp, pr.x, pr.y, MA, MB, c11, c12, c21,
                                          c22);
           else
@@ -1236,10 +1343,14 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // species at the next pass's pair, loaded before the refresh so the
       // loads overlap it (sp at pr.x, pr.y may be stale here: replaced below)
       int q1 = 0, q2 = 0;
-      if (!GUIDED && nxt) {
+      if (!GUIDED && nxt && tid < dw) {
         q1 = sp[nx1];
         q2 = sp[nx2];
       }
+      // plain mode with records: the neighbours whose records change, loaded
+      // before the refresh so the loads overlap it
+      SwapNbrs nbs{{-1, -1, -1}};
+      if (!GUIDED && use_rec) nbs = swap_nbrs_load(A, pr, tid, nthr);
       if constexpr (GUIDED)
         guided_apply_swap<KMAX>(A, T, hist, hl, pr, a, b, tid, nthr,
                                 swap_nbrs_load(A, pr, tid, nthr));
@@ -1247,11 +1358,19 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         const int s = e / (K * K), i = (e / K) % K, j = e % K;
         refresh_entry<KMAX, GUIDED>(A, T, s, i, j);
       }
-      if (!GUIDED && nxt) {  // validity of the next pass after this swap
+      if (!GUIDED && use_rec) guided_apply_swap<KMAX, false>(A, T, rec, hl, pr, a, b, tid, nthr, nbs);
+      if (!GUIDED && nxt) {  // species and validity of the next pass after this swap
         const int v1 = (nx1 == (uint32_t)pr.x) ? b : (nx1 == (uint32_t)pr.y) ? a : q1;
         const int v2 = (nx2 == (uint32_t)pr.x) ? b : (nx2 == (uint32_t)pr.y) ? a : q2;
-        nvalid = v1 != v2;
-        const unsigned long long nb = __ballot(nvalid);
+        bool nv;
+        if constexpr (SP_LDS) {
+          nvalid = v1 != v2;  // every lane draws
+          nv = nvalid;
+        } else {
+          npk = (tid < dw) ? (v1 | (v2 << 8)) : 0;
+          nv = (npk & 255) != (npk >> 8);
+        }
+        const unsigned long long nb = __ballot(nv);
         if (lane == 0) T.sh->wcount[wave] = __popcll(nb);
         nbal_ok = true;
       }
@@ -1463,6 +1582,20 @@ int hea_mc_split(HeaMcArgs& a, bool self) {
   }
   for (int s = a.S; s <= HEA_MAX_S; ++s) a.gbase[s] = g;
   return g;
+}
+
+// Draw lanes per pass when species are read from HBM (every draw then costs
+// two species loads): enough that one pass finds `task` valid draws with a
+// wide margin, p = 1 - sum_i c_i^2 being the chance that two random sites
+// hold different species (about 13 standard deviations to spare for 256
+// proposals of six species); a short pass only costs another pass.
+int hea_mc_draw_lanes(int task, const double* conc, int K, int lanes) {
+  double q = 0.0;
+  for (int i = 0; i < K; ++i) q += conc[i] * conc[i];
+  const double p = std::max(1e-3, 1.0 - q);
+  const double need = (task + 4.0 * std::sqrt(task / p) + 16.0) / p;
+  const int dw = (int)std::min(1e9, std::ceil(need / 64.0)) * 64;
+  return std::max(64, std::min(dw, lanes));
 }
 
 // Table width of the kernel chosen for K species (see mc_fn_for).
