@@ -727,6 +727,26 @@ __device__ __forceinline__ SwapNbrs swap_nbrs_load(const HeaMcArgs& A, int2 pr, 
   return r;
 }
 
+// swap_nbrs_load from copies of the two sites' rows (a1's, then a2's, each
+// A.stride entries) kept in LDS.
+__device__ __forceinline__ SwapNbrs swap_nbrs_rows(const HeaMcArgs& A, const int32_t* rows,
+                                                   int first, int stride) {
+  int zs = 0;
+  for (int s = 0; s < A.S; ++s) zs += A.Z[s];
+  SwapNbrs r;
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int t = first + u * stride;
+    r.nb[u] = -1;
+    if (t < 2 * zs) {
+      int q = t < zs ? t : t - zs, s = 0;
+      while (q >= A.Z[s]) q -= A.Z[s++];
+      r.nb[u] = rows[(t < zs ? 0 : A.stride) + A.off[s] + q];
+    }
+  }
+  return r;
+}
+
 // After swapping a1 = pr.x (species a -> b) and a2 = pr.y (b -> a): every
 // neighbour of the two sites (n, from swap_nbrs_load with the same first and
 // stride) sees the change in its neighbour-count record (the relation is
@@ -967,6 +987,7 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   uint32_t nx1 = 0, nx2 = 0;
   bool nxt = false;        // nx1, nx2 hold the pair at pos + tid
   int npk = 0;             // species in HBM: its species a | b << 8 (valid iff they differ)
+  int nsq = 0;             // species in HBM: its species before this round's swap
   bool nvalid = false;     // species in LDS: its validity (both when nbal_ok)
   bool nbal_ok = false;    // this wave's ballot of the pass is in wcount already
   // draw lanes per pass: all lanes when species sit in LDS; with species in
@@ -974,6 +995,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   // two species loads)
   const int dw = SP_LDS ? nthr : max(64, min(A.dw > 0 ? A.dw : nthr, nthr));
   int2* dq = reinterpret_cast<int2*>(T.cmap);  // [64] (cmap is unused in plain mode)
+  // plain mode with records: the probable winner's index and the two rows of
+  // its sites, fetched during the exact stage (cmap words 128 and 132..)
+  int* pj = reinterpret_cast<int*>(T.cmap + 128);
+  int32_t* prow = reinterpret_cast<int32_t*>(T.cmap + 132);  // [2][stride]
+  if (!GUIDED && tid == 0) *pj = -1;
 
   constexpr bool guided = GUIDED;
   // guided mode: hist[k] = neighbour species counts of site k (HeaHistLayout:
@@ -1103,7 +1129,10 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         valid = SP_LDS ? nvalid : (npk & 255) != (npk >> 8);
         bal = __ballot(valid);
       } else {
-        ab = (tid < dw) ? (int)sp[s1] | ((int)sp[s2] << 8) : 0;
+        if (!SP_LDS && first && nxt)  // loaded during the exact stage, no swap since
+          ab = (tid < dw) ? nsq : 0;
+        else
+          ab = (tid < dw) ? (int)sp[s1] | ((int)sp[s2] << 8) : 0;
         valid = (ab & 255) != (ab >> 8);
         bal = __ballot(valid);
         if (lane == 0) T.sh->wcount[wave] = __popcll(bal);
@@ -1308,12 +1337,46 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       exact_stage<KMAX, EXACT>(A, T, sp, lane, task, nsup, xmin, beta, true);
       __builtin_amdgcn_s_setprio(0);
     } else if constexpr (!GUIDED) {
+      // with species in HBM their loads for the next pass are issued here
+      // too (an accept below replaces the two swapped sites' species)
       hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pos + (uint32_t)tid, (uint32_t)N, &nx1,
                       &nx2);
-      if (wave == 1) {
+      if constexpr (!SP_LDS)
+        if (tid < dw) nsq = (int)sp[nx1] | ((int)sp[nx2] << 8);
+      if (EXACT && use_rec && wave == 2) {  // (not in the batch kernels: registers)
+        // the probable winner, the candidate with the smallest X (the first
+        // in j among equals): its rows for the accept's record updates (the
+        // exact stage compacts cl meanwhile; any entry read is a proposal)
+        const int ncl = T.sh->ncl;
+        double bx = __builtin_inf();
+        int bj = 0x7fffffff;
+        for (int k = lane; k < ncl; k += 64) {
+          const int j = T.cl[k];
+          const double x = T.xs[j];
+          if (x < bx || (x == bx && j < bj)) {
+            bx = x;
+            bj = j;
+          }
+        }
+        const double m = wave_min(bx);
+        const int js = (int)wave_min((bx == m) ? (double)bj : 2147483647.0);
+        if (js < task) {
+          const int2 pr = T.prop[js];
+          const int rch = A.stride >> 2;
+          for (int t = lane; t < 2 * rch; t += 64)
+            reinterpret_cast<int4*>(prow)[t] = reinterpret_cast<const int4*>(
+                A.nbr + (size_t)(t < rch ? pr.x : pr.y) * A.stride)[t < rch ? t : t - rch];
+        }
+        if (lane == 0) *pj = js < task ? js : -1;
+      }
+      if (wave == 1) {  // wave 0's share: species in bits 26.. of the sites (N <= 2^26)
         uint32_t u1, u2;
         hea_stream_pair(A.seed, iter, A.phase, (uint32_t)l, pos + (uint32_t)lane, (uint32_t)N,
                         &u1, &u2);
+        if constexpr (!SP_LDS) {
+          u1 |= (uint32_t)sp[u1] << 26;
+          u2 |= (uint32_t)sp[u2] << 26;
+        }
         dq[lane] = make_int2((int)u1, (int)u2);
       }
     }
@@ -1323,8 +1386,9 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     if (!GUIDED && nthr > 64) {
       if (wave == 0) {
         const int2 v = dq[lane];
-        nx1 = (uint32_t)v.x;
-        nx2 = (uint32_t)v.y;
+        nx1 = (uint32_t)v.x & 0x3ffffffu;
+        nx2 = (uint32_t)v.y & 0x3ffffffu;
+        if constexpr (!SP_LDS) nsq = (int)((uint32_t)v.x >> 26) | ((int)((uint32_t)v.y >> 26) << 8);
       }
       nxt = true;
     }
@@ -1343,14 +1407,21 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       // species at the next pass's pair, loaded before the refresh so the
       // loads overlap it (sp at pr.x, pr.y may be stale here: replaced below)
       int q1 = 0, q2 = 0;
-      if (!GUIDED && nxt && tid < dw) {
-        q1 = sp[nx1];
-        q2 = sp[nx2];
+      if constexpr (SP_LDS) {
+        if (!GUIDED && nxt) {
+          q1 = sp[nx1];
+          q2 = sp[nx2];
+        }
+      } else {  // loaded during the exact stage
+        q1 = nsq & 255;
+        q2 = nsq >> 8;
       }
       // plain mode with records: the neighbours whose records change, loaded
       // before the refresh so the loads overlap it
       SwapNbrs nbs{{-1, -1, -1}};
-      if (!GUIDED && use_rec) nbs = swap_nbrs_load(A, pr, tid, nthr);
+      if (!GUIDED && use_rec)
+        nbs = (EXACT && *pj == T.sh->res_j) ? swap_nbrs_rows(A, prow, tid, nthr)
+                                   : swap_nbrs_load(A, pr, tid, nthr);
       if constexpr (GUIDED)
         guided_apply_swap<KMAX>(A, T, hist, hl, pr, a, b, tid, nthr,
                                 swap_nbrs_load(A, pr, tid, nthr));
