@@ -128,3 +128,25 @@ def test_fcc50_properties(gpu, guided):
     _check_population(spec, p, o, lat, fit, st, run["parallel_task"], sample=2, guided=guided)
     lat2, fit2, _ = _session(p, run, 8, guide=g)
     assert np.array_equal(lat, lat2) and np.array_equal(_bits(fit), _bits(fit2))
+
+
+def test_fcc50_configured_launches(gpu):
+    """FCC 50^3 as configured: 64 solutions, the guided search in one
+    persistent launch per phase (8 candidates, tau 0.1, up to 8 accepts per
+    round) and the plain session, both with the neighbour-records walk; one
+    global iteration each."""
+    spec, run = workload(4)
+    o, p = Oracle.from_spec(spec), Problem.from_spec(spec)
+    L, T = run["solutions"], run["parallel_task"]
+    b = Batch([p], L, T, run["converge_depth"], seed=run["seed"])
+    b.guide(candidates=8, tau=0.1, accepts=8)
+    b.init()
+    b.step(1)
+    lat, fit, st = b.read(0)
+    b.close()
+    assert lat.shape == (L, spec.n_sites)
+    _check_population(spec, p, o, lat, fit, st, T, sample=2, guided=True)
+    assert st["accepts"] > 8 * L
+    lat, fit, st = _session(p, run, L)
+    _check_population(spec, p, o, lat, fit, st, T, sample=2)
+    assert st["accepts"] > L
