@@ -768,7 +768,10 @@ __device__ __forceinline__ void guided_apply_swap(const HeaMcArgs& A, const Lds<
     int q = t < zs ? t : t - zs, s = 0;
     while (q >= A.Z[s]) q -= A.Z[s++];
     // the last species has no field (its count is implied)
-    const int os = s == 0 ? hl.o[0] : s == 1 ? hl.o[1] : hl.o[2];  // no variable index
+    // the plain kernels select the field without a variable index (which puts
+    // the layout in scratch memory); for the guided ones the indexed form
+    // measured 1.3 % faster
+    const int os = MAP ? hl.o[s] : s == 0 ? hl.o[0] : s == 1 ? hl.o[1] : hl.o[2];
     const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + os) : 0ull;
     const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + os) : 0ull;
     const unsigned long long dv = t < zs ? fb - fa : fa - fb;  // modulo 2^64
@@ -1020,13 +1023,21 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
     for (int k = tid; k < N; k += nthr) {
       const int32_t* row = A.nbr + (size_t)k * A.stride;
       unsigned long long m = 0ull;
-#pragma unroll
-      for (int s = 0; s < HEA_MAX_S; ++s)
-        if (s < S)
+      if constexpr (GUIDED) {
+        for (int s = 0; s < S; ++s)
           for (int q = 0; q < A.Z[s]; ++q) {
             const int c = sp[row[A.off[s] + q]];
             if (c < K - 1) m += 1ull << (c * hl.W + hl.o[s]);
           }
+      } else {  // (without a variable index into the layout: see guided_apply_swap)
+#pragma unroll
+        for (int s = 0; s < HEA_MAX_S; ++s)
+          if (s < S)
+            for (int q = 0; q < A.Z[s]; ++q) {
+              const int c = sp[row[A.off[s] + q]];
+              if (c < K - 1) m += 1ull << (c * hl.W + hl.o[s]);
+            }
+      }
       rb[k] = m;
     }
     __syncthreads();  // every site's record before the first read
