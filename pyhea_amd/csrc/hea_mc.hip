@@ -1459,9 +1459,22 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       __syncthreads();
       if constexpr (GUIDED) {
         if (A.g_multi > 1) {
-          if (wave == 0)
+          if (wave == 0) {
             accepts += guided_more_accepts<KMAX, EXACT>(A, T, sp, hist, hl, lane, task,
                                                         A.g_multi - 1, &screens, &hit_thr);
+          } else if (wave == 1) {
+            // warm this CU's L1 with the rows of the shortlisted proposals'
+            // sites, which the further accepts load for their record updates
+            const int j = T.sl[lane];
+            if (j >= 0) {
+              const int2 pr = T.prop[j];
+              const int4* r1 = reinterpret_cast<const int4*>(A.nbr + (size_t)pr.x * A.stride);
+              const int4* r2 = reinterpret_cast<const int4*>(A.nbr + (size_t)pr.y * A.stride);
+              int acc = 0;
+              for (int c = 0; c < (A.stride >> 2); c += 2) acc ^= r1[c].x ^ r2[c].x;
+              asm volatile("" ::"v"(acc));
+            }
+          }
           __syncthreads();
         }
       }
