@@ -1554,7 +1554,13 @@ static int batch_records(hea_batch b) {
     HEA_HIP(b->hist.alloc(n));
   }
   unsigned long long* base = b->hist.p;
-  return batch_patch_args(b, [&](HeaMcArgs& a, int q) { a.rec = use[q] ? base + off[q] : nullptr; });
+  b->threads = 0;  // the screen lanes depend on the walk (hea_mc_split)
+  return batch_patch_args(b, [&](HeaMcArgs& a, int q) {
+    a.rec = use[q] ? base + off[q] : nullptr;
+    const int lanes = hea_mc_split(a, false);
+    a.dw = hea_mc_draw_lanes(b->task, b->probs[q]->hc.conc, b->probs[q]->hc.K, lanes);
+    b->threads = std::max(b->threads, lanes);
+  });
 }
 
 int hea_batch_tile(hea_batch b, int32_t tp) {

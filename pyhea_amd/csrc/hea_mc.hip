@@ -1649,12 +1649,13 @@ hipError_t hea_launch_local_mc_batch(const HeaMcBatchArgs& b, int kmax, bool sp_
 // each) of the next-longest list, into parts no shorter than that list: the
 // long shell then stops keeping the others waiting, and shorter gains do not
 // pay for the extra lanes (measured: FCC 12/6/24 gains, BCC 8/6/12 loses).
-// Tiny cells whose sites list themselves (self) keep whole lists.  Returns
-// the lanes.
+// Tiny cells whose sites list themselves (self) keep whole lists, and so do
+// walks that read neighbour records (a.rec): their lanes only scan a1's
+// row, and unsplit lists measured 3 % faster at FCC 50^3.  Returns the lanes.
 int hea_mc_split(HeaMcArgs& a, bool self) {
   int nch[HEA_MAX_S], lg[HEA_MAX_S] = {0, 0, 0};
   for (int s = 0; s < a.S; ++s) nch[s] = std::max(1, (a.Z[s] + 3) >> 2);
-  for (int s = 0; s < a.S && !self; ++s) {
+  for (int s = 0; s < a.S && !self && !a.rec; ++s) {
     int next = 0;  // longest other list
     for (int t = 0; t < a.S; ++t)
       if (t != s) next = std::max(next, nch[t]);
