@@ -134,3 +134,25 @@ def test_guided_species_in_hbm_matches_lds(gpu, monkeypatch, case, accepts):
         assert st["evals"] == s0["evals"] and st["accepts"] == s0["accepts"]
     assert np.allclose(o.fitness(l0), f0, rtol=1e-11, atol=0)
     assert s0["accepts"] > 0
+
+
+def test_batch_mixed_walks_species_in_hbm(gpu, monkeypatch):
+    """A batch holding a 7-species problem runs the width-8 kernel; its
+    problems with at most 6 species still use records (8-bit packed walk
+    histograms), the 7-species one gathers.  Each equals its plain search."""
+    cases = [("FCC", 3, [0.15] * 6 + [0.1], "true", 3), ("BCC", 4, [0.5, 0.25, 0.25], "true", 3),
+             ("FCC", 3, [0.3, 0.2, 0.2, 0.1, 0.1, 0.1], "true", 3)]
+    specs = [_spec(c, seed=80 + i) for i, c in enumerate(cases)]
+    hbm = [_problem(monkeypatch, s) for s in specs]
+    lds = [_problem(monkeypatch, s, hbm=False) for s in specs]
+    L, I, T, D, seed = 4, 2, 40, 2, 650
+    b = Batch(hbm, L, T, D, seed=seed)
+    b.init()
+    b.step(I)
+    for q, p in enumerate(lds):
+        lb, fb, sb = b.read(q)
+        lp, fp, sp = p.run_hcs(L, I, T, D, seed=seed + q)
+        assert np.array_equal(lb, lp), q
+        assert np.array_equal(_bits(fb), _bits(fp)), q
+        assert sb["evals"] == sp["evals"] and sb["accepts"] == sp["accepts"], q
+    b.close()
