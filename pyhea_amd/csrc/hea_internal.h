@@ -69,9 +69,9 @@ struct HeaMcArgs {
   // then returns without searching
   int32_t* dbg_pairs;
   float* dbg_pred;
-  // neighbour records for the screen walk when species are read from HBM
-  // (nullptr: the walk gathers every neighbour's species).  Guided mode
-  // keeps its records in hist and sets rec = hist to use them.
+  // neighbour records for the plain screen walk when species are read from
+  // HBM (nullptr: the walk gathers every neighbour's species).  Guided mode
+  // always walks with its own records (hist).
   unsigned long long* rec;      // [L][N] (HeaHistLayout)
   // problem scalars, copied from HeaConst so they live in the kernarg segment
   int32_t N, K, S, stride;
@@ -99,8 +99,9 @@ struct HeaMcBatchArgs {
 // the field at bit x*W + o[s] of width w[s] (the bit width of Z_s; W = sum of
 // the widths); species K-1's count is Z_s minus the others.  Usable when
 // (K-1)*W <= 64 (FCC 12/6/24 needs 12 bits per species, BCC 8/6/12 11).
-// Guided mode scores candidates from them; when species are read from HBM
-// the screen walk takes its species histograms from them as well (exact:
+// Guided mode scores candidates from them and its screen walk takes its
+// species histograms from them; so does the plain walk when species are
+// read from HBM (exact:
 // the neighbour relation is symmetric, checked when a problem is created,
 // so a swap changes exactly the records of the two sites' neighbours).
 struct HeaHistLayout {

@@ -1009,10 +1009,11 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
   // 8 bytes per site, so the records of many lattices stay cache-resident)
   unsigned long long* hist = guided ? A.hist + (size_t)l * N : nullptr;
   const HeaHistLayout hl = hea_hist_layout(S, A.Z);
-  // species read from HBM: the screen walk takes its species histograms from
-  // the neighbour records (guided mode's own, or this lattice's slice of
-  // A.rec) instead of gathering every neighbour's species
-  const bool use_rec = !SP_LDS && !SELF && A.rec != nullptr;
+  // the screen walk takes its species histograms from the neighbour records
+  // instead of gathering every neighbour's species: always in guided mode,
+  // which keeps the records anyway (the walk then reads one row per
+  // proposal instead of two), and with species in HBM when A.rec is set
+  const bool use_rec = GUIDED || (!SP_LDS && !SELF && A.rec != nullptr);
   unsigned long long* rec = use_rec ? (GUIDED ? hist : A.rec + (size_t)l * N) : nullptr;
   // this wave's shell fields in the records (selected without indexing the
   // layout by a variable, which would move it to scratch memory)

@@ -120,9 +120,10 @@ def _guided(p, accepts, L=6, I=2, T=64, D=3, seed=5):
 @pytest.mark.parametrize("accepts", [1, 4])
 @pytest.mark.parametrize("case", [CASES[1], CASES[3]], ids=_id)
 def test_guided_species_in_hbm_matches_lds(gpu, monkeypatch, case, accepts):
-    """Guided search: species in LDS, in HBM with the records walk and in HBM
-    with the gathering walk make the same decisions (the walk's histograms are
-    the same integers), and the stored fitness is the oracle's recount."""
+    """Guided search (which always walks with its records): species in LDS
+    and in HBM make the same decisions, with or without the switch that
+    makes the plain walk gather, and the stored fitness is the oracle's
+    recount."""
     spec = _spec(case)
     o = Oracle.from_spec(spec)
     runs = [_guided(_problem(monkeypatch, spec, hbm=h, walk=w), accepts)
