@@ -111,7 +111,15 @@ int hea_local_mc(hea_problem p, int32_t solutions, uint8_t* lattices, double* fi
 
 /* Same on device-resident buffers (d_lattices [solutions][N] uint8,
  * d_fitness [solutions] double) on `stream` (a hipStream_t, NULL = default);
- * asynchronous: no host synchronisation. */
+ * asynchronous: no host synchronisation.
+ *
+ * Device memory: when a lattice's species do not fit in a workgroup's LDS
+ * (beyond about 110000 sites, depending on parallel_task and K) the search
+ * reads them from HBM and keeps one 64-bit neighbour-count record per site
+ * and solution (8 * solutions * N bytes) for its walk; hea_local_mc and
+ * hea_local_mc_device grow the problem's buffer for it on demand, sessions
+ * and batches allocate theirs when created (or when a tile or guide call
+ * changes the placement).  Results are the same with or without it. */
 int hea_local_mc_device(hea_problem p, int32_t solutions, void* d_lattices, void* d_fitness,
                         int32_t parallel_task, int32_t converge_depth, uint64_t seed,
                         uint32_t iter, uint32_t phase, void* stream);
