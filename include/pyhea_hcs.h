@@ -219,7 +219,8 @@ int hea_session_destroy(hea_session s);
  * draws from seed `seed + p`, so its population equals
  * hea_hcs_run(probs[p], ..., seed + p, ...).  Problems must share the
  * device, the shell count and have no self neighbours; parallel_task and
- * converge_depth are common.  The problems must outlive the batch. */
+ * converge_depth are common; at most 65535 problems.  The problems must
+ * outlive the batch. */
 int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t solutions,
                      int32_t parallel_task, int32_t converge_depth, double thr, uint64_t seed,
                      hea_batch* out);

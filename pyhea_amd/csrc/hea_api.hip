@@ -239,10 +239,11 @@ __global__ void k_fitness(const HeaConst* __restrict__ pc, const int32_t* __rest
 // order (std::sort of the positions by fitness, accelerate.cpp:37-64) as
 // long as no two kept keys are equal; a kept key that equals another raises
 // *tie, and k_tie_sort then redoes the order with the reference's algorithm.
-__global__ void k_rank(const double* __restrict__ fa, int na, const double* __restrict__ fb,
-                       int nb, int keep, int32_t* __restrict__ perm, uint32_t* __restrict__ tie) {
+__device__ __forceinline__ void rank_one(const double* __restrict__ fa, int na,
+                                         const double* __restrict__ fb, int nb, int keep,
+                                         int32_t* __restrict__ perm, uint32_t* __restrict__ tie,
+                                         int i) {
   const int n = na + nb;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double fi = i < na ? fa[i] : fb[i - na];
   int r = 0;
@@ -258,17 +259,22 @@ __global__ void k_rank(const double* __restrict__ fa, int na, const double* __re
   }
 }
 
+__global__ void k_rank(const double* __restrict__ fa, int na, const double* __restrict__ fb,
+                       int nb, int keep, int32_t* __restrict__ perm, uint32_t* __restrict__ tie) {
+  rank_one(fa, na, fb, nb, keep, perm, tie, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Population order on equal keys: when k_rank saw a tie, one lane sorts the
 // positions 0..n-1 with the reference library's std::sort algorithm
 // (hea_sort.h), in LDS when `lds` holds the keys and positions (12 bytes
 // each), else in place in perm (n entries) reading the keys from HBM; then
 // clears *tie.  Without a tie the kernel returns at once.
-__global__ __launch_bounds__(256) void k_tie_sort(const double* __restrict__ fa, int na,
-                                                  const double* __restrict__ fb, int nb, int keep,
-                                                  int32_t* __restrict__ perm,
-                                                  uint32_t* __restrict__ tie, int lds) {
+__device__ __forceinline__ void tie_sort_one(const double* __restrict__ fa, int na,
+                                             const double* __restrict__ fb, int nb, int keep,
+                                             int32_t* __restrict__ perm,
+                                             uint32_t* __restrict__ tie, int lds,
+                                             unsigned char* smem) {
   if (__builtin_amdgcn_readfirstlane((int)*tie) == 0) return;
-  extern __shared__ __align__(16) unsigned char smem[];
   const int n = na + nb;
   const bool in_lds = lds > 0;
   double* key = reinterpret_cast<double*>(smem);
@@ -289,12 +295,23 @@ __global__ __launch_bounds__(256) void k_tie_sort(const double* __restrict__ fa,
   if (threadIdx.x == 0) *tie = 0u;
 }
 
-// dst row r = (perm[r] < na ? srcA[perm[r]] : srcB[perm[r]-na]).
-__global__ void k_gather(const uint8_t* __restrict__ srcA, const double* __restrict__ fA, int na,
-                         const uint8_t* __restrict__ srcB, const double* __restrict__ fB,
-                         const int32_t* __restrict__ perm, uint8_t* __restrict__ dst,
-                         double* __restrict__ fdst, int N) {
-  const int r = blockIdx.y;
+__global__ __launch_bounds__(256) void k_tie_sort(const double* __restrict__ fa, int na,
+                                                  const double* __restrict__ fb, int nb, int keep,
+                                                  int32_t* __restrict__ perm,
+                                                  uint32_t* __restrict__ tie, int lds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  tie_sort_one(fa, na, fb, nb, keep, perm, tie, lds, smem);
+}
+
+// dst row r = (perm[r] < na ? srcA[perm[r]] : srcB[perm[r]-na]); chunk
+// blockIdx.x of gridDim.x.
+__device__ __forceinline__ void gather_row(const uint8_t* __restrict__ srcA,
+                                           const double* __restrict__ fA, int na,
+                                           const uint8_t* __restrict__ srcB,
+                                           const double* __restrict__ fB,
+                                           const int32_t* __restrict__ perm,
+                                           uint8_t* __restrict__ dst, double* __restrict__ fdst,
+                                           int N, int r) {
   const int p = perm[r];
   const uint8_t* src = p < na ? srcA + (size_t)p * N : srcB + (size_t)(p - na) * N;
   uint8_t* d = dst + (size_t)r * N;
@@ -308,6 +325,44 @@ __global__ void k_gather(const uint8_t* __restrict__ srcA, const double* __restr
       d[i] = src[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) fdst[r] = p < na ? fA[p] : fB[p - na];
+}
+
+__global__ void k_gather(const uint8_t* __restrict__ srcA, const double* __restrict__ fA, int na,
+                         const uint8_t* __restrict__ srcB, const double* __restrict__ fB,
+                         const int32_t* __restrict__ perm, uint8_t* __restrict__ dst,
+                         double* __restrict__ fdst, int N) {
+  gather_row(srcA, fA, na, srcB, fB, perm, dst, fdst, N, blockIdx.y);
+}
+
+// One population merge of every problem of a batch (the three kernels above,
+// problem q in blockIdx.y, blockIdx.x and blockIdx.z respectively).
+struct HeaMergeDesc {
+  const uint8_t* A;
+  const double* fa;
+  const uint8_t* B;
+  const double* fb;
+  int32_t* perm;
+  uint32_t* tie;
+  uint8_t* dst;
+  double* fdst;
+  int32_t na, nb, keep, N;
+};
+
+__global__ void k_rank_batch(const HeaMergeDesc* __restrict__ md) {
+  const HeaMergeDesc m = md[blockIdx.y];
+  rank_one(m.fa, m.na, m.fb, m.nb, m.keep, m.perm, m.tie, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_tie_sort_batch(const HeaMergeDesc* __restrict__ md,
+                                                        int lds) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const HeaMergeDesc m = md[blockIdx.x];
+  tie_sort_one(m.fa, m.na, m.fb, m.nb, m.keep, m.perm, m.tie, lds, smem);
+}
+
+__global__ void k_gather_batch(const HeaMergeDesc* __restrict__ md) {
+  const HeaMergeDesc m = md[blockIdx.z];
+  gather_row(m.A, m.fa, m.na, m.B, m.fb, m.perm, m.dst, m.fdst, m.N, blockIdx.y);
 }
 
 __global__ void k_iter_add(uint32_t* it, uint32_t v) { *it += v; }
@@ -513,6 +568,20 @@ int launch_merge(const uint8_t* A, const double* fa, int na, const uint8_t* B, c
   const unsigned gx = (unsigned)std::max(1, std::min(64, (per + 255) / 256));
   hipLaunchKernelGGL(k_gather, dim3(gx, keep), dim3(256), 0, st, A, fa, na, B, fb, perm, dst,
                      fdst, N);
+  HEA_HIP(hipGetLastError());
+  return HEA_OK;
+}
+
+// launch_merge for P problems at once: md[q] is problem q's merge; every
+// problem merges na + nb = n rows into keep, its lattices at most maxN sites.
+int launch_merge_batch(const HeaMergeDesc* md, int P, int n, int keep, int maxN, hipStream_t st) {
+  hipLaunchKernelGGL(k_rank_batch, dim3((n + 255) / 256, P), dim3(256), 0, st, md);
+  const size_t tl = (size_t)n * (sizeof(double) + sizeof(int));
+  const int lds = tl <= 65536 ? (int)tl : 0;
+  hipLaunchKernelGGL(k_tie_sort_batch, dim3(P), dim3(256), (size_t)lds, st, md, lds);
+  const int per = (maxN % 16 == 0) ? maxN / 16 : maxN;
+  const unsigned gx = (unsigned)std::max(1, std::min(64, (per + 255) / 256));
+  hipLaunchKernelGGL(k_gather_batch, dim3(gx, keep, P), dim3(256), 0, st, md);
   HEA_HIP(hipGetLastError());
   return HEA_OK;
 }
@@ -1305,6 +1374,9 @@ struct hea_batch_s {
   DevBuf<uint32_t> tie;            // [problems] tie flags of the population sorts
   DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
   DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
+  DevBuf<HeaMergeDesc> merges;     // [3][problems]: B -> C and A ++ C -> B after local
+                                   // search I, B -> A after local search II
+  int maxN = 0;
   DevBuf<int2> items;
   DevBuf<unsigned long long> hist;  // guided mode, [sum_p L N_p]
   GuideCfg guide;
@@ -1352,24 +1424,15 @@ static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
       HEA_HIP(hipMemcpyAsync(hk->snap + (h + 1) * sn, b->stats.p, sizeof(int64_t) * sn,
                              hipMemcpyDeviceToDevice, st));
     }
-    for (int q = 0; q < P; ++q) {
-      hea_problem p = b->probs[q];
-      const int N = p->hc.N;
-      uint8_t* A = b->A.p + b->lat_off[q];
-      uint8_t* B = b->B.p + b->lat_off[q];
-      uint8_t* C = b->C.p + b->lat_off[q];
-      double* fA = b->fA.p + (size_t)q * L;
-      double* fB = b->fB.p + (size_t)q * L;
-      double* fC = b->fC.p + (size_t)q * L;
-      int32_t* perm = b->perm.p + (size_t)q * 2 * L;
-      uint32_t* tie = b->tie.p + q;
-      if (phase == HEA_PHASE_LS1) {
-        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, tie, C, fC, N, st))) return rc;
-        if ((rc = launch_merge(A, fA, L, C, fC, L, L, perm, tie, B, fB, N, st))) return rc;
-        if ((rc = launch_recount(p, B, b->counts.p + b->cnt_off[q], L, st))) return rc;
-      } else {
-        if ((rc = launch_merge(B, fB, L, B, fB, 0, L, perm, tie, A, fA, N, st))) return rc;
-      }
+    if (phase == HEA_PHASE_LS1) {
+      if ((rc = launch_merge_batch(b->merges.p, P, L, L, b->maxN, st))) return rc;
+      if ((rc = launch_merge_batch(b->merges.p + P, P, 2 * L, L, b->maxN, st))) return rc;
+      for (int q = 0; q < P; ++q)
+        if ((rc = launch_recount(b->probs[q], b->B.p + b->lat_off[q], b->counts.p + b->cnt_off[q],
+                                 L, st)))
+          return rc;
+    } else {
+      if ((rc = launch_merge_batch(b->merges.p + 2 * P, P, L, L, b->maxN, st))) return rc;
     }
   }
   hipLaunchKernelGGL(k_iter_add, dim3(1), dim3(1), 0, st, b->iter.p, 1u);
@@ -1384,6 +1447,8 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (!out) return fail(HEA_EINVAL, "null output handle");
   *out = nullptr;
   if (!probs || n_probs < 1) return fail(HEA_EINVAL, "need at least one problem");
+  if (n_probs > 65535)  // the batched merge kernels index problems by a grid dimension
+    return fail(HEA_EINVAL, "at most 65535 problems per batch");
   int rc;
   for (int q = 0; q < n_probs; ++q)
     if ((rc = check_run_args(probs[q], L, task, depth))) return rc;
@@ -1463,6 +1528,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (e == hipSuccess) e = b->queue.alloc(HEA_XCDS);
   if (e == hipSuccess) e = b->clk.alloc((size_t)1 + n_probs);
   if (e == hipSuccess) e = b->margs.alloc((size_t)2 * n_probs);
+  if (e == hipSuccess) e = b->merges.alloc((size_t)3 * n_probs);
   if (e == hipSuccess) e = b->items.alloc(items.size());
   if (e == hipSuccess) e = b->tie.alloc((size_t)n_probs);
   if (e == hipSuccess) e = hipMemset(b->tie.p, 0, sizeof(uint32_t) * n_probs);
@@ -1493,7 +1559,26 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
       a.seed = seed + (uint64_t)q;
       a.phase = (uint32_t)(HEA_PHASE_LS1 + ph);
     }
+  b->maxN = maxN;
+  std::vector<HeaMergeDesc> md((size_t)3 * n_probs);
+  for (int q = 0; q < n_probs; ++q) {
+    uint8_t* A = b->A.p + b->lat_off[q];
+    uint8_t* B = b->B.p + b->lat_off[q];
+    uint8_t* C = b->C.p + b->lat_off[q];
+    double* fA = b->fA.p + (size_t)q * L;
+    double* fB = b->fB.p + (size_t)q * L;
+    double* fC = b->fC.p + (size_t)q * L;
+    int32_t* perm = b->perm.p + (size_t)q * 2 * L;
+    uint32_t* tie = b->tie.p + q;
+    const int N = probs[q]->hc.N;
+    md[q] = HeaMergeDesc{B, fB, B, fB, perm, tie, C, fC, L, 0, L, N};
+    md[(size_t)n_probs + q] = HeaMergeDesc{A, fA, C, fC, perm, tie, B, fB, L, L, L, N};
+    md[(size_t)2 * n_probs + q] = HeaMergeDesc{B, fB, B, fB, perm, tie, A, fA, L, 0, L, N};
+  }
   e = hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * ma.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->merges.p, md.data(), sizeof(HeaMergeDesc) * md.size(),
+                  hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds);
