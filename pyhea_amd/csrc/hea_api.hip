@@ -73,18 +73,24 @@ unsigned char* scratch_of(int device) {
 
 // ------------------------------------------------------------------ kernels --
 
-// lattice[l][i] = sorted_multiset[perm_l(i)]
+// lattice[l][i] = sorted_multiset[perm_l(i)] for i = i0, i0 + di, ... < N.
+__device__ __forceinline__ void permute_sites(const HeaConst& P, uint8_t* __restrict__ out,
+                                              uint64_t seed, uint32_t iter, uint32_t phase,
+                                              int slot, int l, int i0, int di) {
+  if (i0 >= P.N) return;
+  const hea_perm_key key = hea_perm_make_key(seed, iter, phase, (uint32_t)slot, (uint32_t)P.N);
+  for (int i = i0; i < P.N; i += di) {
+    const uint32_t q = hea_perm_apply(&key, (uint32_t)i);
+    out[(size_t)l * P.N + i] = (uint8_t)hea_species_of_rank(P.prefix, P.K, q);
+  }
+}
+
 __global__ void k_permute(const HeaConst* __restrict__ pc, uint8_t* __restrict__ out,
                           uint64_t seed, const uint32_t* __restrict__ iter_ptr, uint32_t phase,
                           int slot0) {
-  const HeaConst& P = *pc;
   const int l = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.N) return;
-  const hea_perm_key key =
-      hea_perm_make_key(seed, *iter_ptr, phase, (uint32_t)(slot0 + l), (uint32_t)P.N);
-  const uint32_t q = hea_perm_apply(&key, (uint32_t)i);
-  out[(size_t)l * P.N + i] = (uint8_t)hea_species_of_rank(P.prefix, P.K, q);
+  permute_sites(*pc, out, seed, *iter_ptr, phase, slot0 + l, l, i, 1 << 30);
 }
 
 __global__ void k_zero_i32(int32_t* __restrict__ p, size_t n) {
@@ -170,13 +176,11 @@ __global__ __launch_bounds__(kRecountThreads) void k_recount(
 // The same with one workgroup per lattice and the lattice's species staged
 // in LDS; per-wave histogram tables, then one store per entry (no zeroing of
 // `counts` needed).
-__global__ __launch_bounds__(kRecountLdsThreads) void k_recount_lds(
-    const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
-    const uint8_t* __restrict__ lattices, int32_t* __restrict__ counts) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void recount_lds_one(const HeaConst& P, const int32_t* __restrict__ nbr,
+                                                const uint8_t* __restrict__ lattices,
+                                                int32_t* __restrict__ counts, int l,
+                                                unsigned char* smem) {
   constexpr int TAB = HEA_MAX_S * HEA_MAX_K * HEA_MAX_K;
-  const HeaConst& P = *pc;
-  const int l = blockIdx.x;
   const int K = P.K, S = P.S, N = P.N, SKK = S * K * K;
   const int tid = threadIdx.x, wave = tid >> 6, nw = blockDim.x >> 6;
   int32_t* wh = reinterpret_cast<int32_t*>(smem);  // [nw][TAB]
@@ -206,16 +210,21 @@ __global__ __launch_bounds__(kRecountLdsThreads) void k_recount_lds(
   }
 }
 
+__global__ __launch_bounds__(kRecountLdsThreads) void k_recount_lds(
+    const HeaConst* __restrict__ pc, const int32_t* __restrict__ nbr,
+    const uint8_t* __restrict__ lattices, int32_t* __restrict__ counts) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  recount_lds_one(*pc, nbr, lattices, counts, blockIdx.x, smem);
+}
+
 size_t recount_lds_bytes(int n) {
   return sizeof(int32_t) * HEA_MAX_S * HEA_MAX_K * HEA_MAX_K * (kRecountLdsThreads / 64) + n;
 }
 
 // Fitness of freshly recounted lattices: gamma = R_s(n), then the
 // reference's sum (accelerate.cpp:165-178; std::pow(x, 2) there is x*x).
-__global__ void k_fitness(const HeaConst* __restrict__ pc, const int32_t* __restrict__ counts,
-                          double* __restrict__ fit, int L) {
-  const HeaConst& P = *pc;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fitness_one(const HeaConst& P, const int32_t* __restrict__ counts,
+                                            double* __restrict__ fit, int L, int l) {
   if (l >= L) return;
   const int K = P.K, S = P.S;
   const int32_t* c = counts + (size_t)l * S * K * K;
@@ -232,6 +241,45 @@ __global__ void k_fitness(const HeaConst* __restrict__ pc, const int32_t* __rest
     error = error + P.weight[s] * es;
   }
   fit[l] = sqrt(error);
+}
+
+__global__ void k_fitness(const HeaConst* __restrict__ pc, const int32_t* __restrict__ counts,
+                          double* __restrict__ fit, int L) {
+  fitness_one(*pc, counts, fit, L, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// One population of every problem of a batch (problem q in the last grid
+// dimension): its lattices, their pair counts and fitness, and the seed of
+// its keyed permutations.
+struct HeaPopDesc {
+  const HeaConst* pc;
+  const int32_t* nbr;
+  uint8_t* lat;
+  int32_t* counts;
+  double* fit;
+  uint64_t seed;
+};
+
+// k_permute over every problem; blockIdx.x of gridDim.x takes every
+// gridDim.x-th chunk of 256 sites.
+__global__ void k_permute_batch(const HeaPopDesc* __restrict__ pd,
+                                const uint32_t* __restrict__ iter_ptr, uint32_t phase) {
+  const HeaPopDesc d = pd[blockIdx.z];
+  const int l = blockIdx.y;
+  permute_sites(*d.pc, d.lat, d.seed, *iter_ptr, phase, l, l,
+                blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ __launch_bounds__(kRecountLdsThreads) void k_recount_lds_batch(
+    const HeaPopDesc* __restrict__ pd) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const HeaPopDesc d = pd[blockIdx.y];
+  recount_lds_one(*d.pc, d.nbr, d.lat, d.counts, blockIdx.x, smem);
+}
+
+__global__ void k_fitness_batch(const HeaPopDesc* __restrict__ pd, int L) {
+  const HeaPopDesc d = pd[blockIdx.y];
+  fitness_one(*d.pc, d.counts, d.fit, L, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Rank by (fitness, position) over fa[0..na) ++ fb[0..nb); writes
@@ -804,6 +852,9 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
     p->max_lds = lds_max;
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_recount_lds),
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_recount_lds_batch),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
     if (e != hipSuccess) { rc = fail(HEA_EHIP, hipGetErrorString(e)); break; }
     if (p->d_const.alloc(1) != hipSuccess || p->d_nbr.alloc(padded.size()) != hipSuccess) {
       rc = fail(HEA_ENOMEM, "device allocation failed");
@@ -1376,7 +1427,9 @@ struct hea_batch_s {
   DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
   DevBuf<HeaMergeDesc> merges;     // [3][problems]: B -> C and A ++ C -> B after local
                                    // search I, B -> A after local search II
+  DevBuf<HeaPopDesc> pops;         // [problems]: population B, its counts and fitness
   int maxN = 0;
+  size_t recount_lds = 0;          // dynamic LDS of the batched recount (0: per problem)
   DevBuf<int2> items;
   DevBuf<unsigned long long> hist;  // guided mode, [sum_p L N_p]
   GuideCfg guide;
@@ -1385,22 +1438,34 @@ struct hea_batch_s {
 extern "C" int hea_batch_grid(hea_batch b, int32_t grid);  // autotuner hook (below)
 static int batch_records(hea_batch b);
 
+// Pair counts of population B of every problem: one launch when every
+// problem's lattices fit in LDS, else one recount per problem.
+static int batch_recount(hea_batch b) {
+  const int L = b->L, P = (int)b->probs.size();
+  if (b->recount_lds) {
+    hipLaunchKernelGGL(k_recount_lds_batch, dim3(L, P), dim3(kRecountLdsThreads), b->recount_lds,
+                       b->stream, b->pops.p);
+    HEA_HIP(hipGetLastError());
+    return HEA_OK;
+  }
+  int rc;
+  for (int q = 0; q < P; ++q)
+    if ((rc = launch_recount(b->probs[q], b->B.p + b->lat_off[q], b->counts.p + b->cnt_off[q], L,
+                             b->stream)))
+      return rc;
+  return HEA_OK;
+}
+
 static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
   const int L = b->L, P = (int)b->probs.size();
   const size_t sn = (size_t)4 * L * P;  // stats entries of the whole batch
   hipStream_t st = b->stream;
   int rc;
-  for (int q = 0; q < P; ++q) {
-    hea_problem p = b->probs[q];
-    const int N = p->hc.N;
-    dim3 g((N + 255) / 256, L);
-    hipLaunchKernelGGL(k_permute, g, dim3(256), 0, st, p->d_const.p, b->B.p + b->lat_off[q],
-                       b->seed + (uint64_t)q, b->iter.p, (uint32_t)HEA_PHASE_NEW, 0);
-    if ((rc = launch_recount(p, b->B.p + b->lat_off[q], b->counts.p + b->cnt_off[q], L, st)))
-      return rc;
-    if ((rc = launch_fitness(p, b->counts.p + b->cnt_off[q], b->fB.p + (size_t)q * L, L, st)))
-      return rc;
-  }
+  const unsigned gx = (unsigned)std::min(16, (b->maxN + 255) / 256);
+  hipLaunchKernelGGL(k_permute_batch, dim3(gx, L, P), dim3(256), 0, st, b->pops.p, b->iter.p,
+                     (uint32_t)HEA_PHASE_NEW);
+  if ((rc = batch_recount(b))) return rc;
+  hipLaunchKernelGGL(k_fitness_batch, dim3((L + 63) / 64, P), dim3(64), 0, st, b->pops.p, L);
   HeaMcBatchArgs ba;
   ba.items = b->items.p;
   ba.n_items = P * L;
@@ -1427,10 +1492,7 @@ static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
     if (phase == HEA_PHASE_LS1) {
       if ((rc = launch_merge_batch(b->merges.p, P, L, L, b->maxN, st))) return rc;
       if ((rc = launch_merge_batch(b->merges.p + P, P, 2 * L, L, b->maxN, st))) return rc;
-      for (int q = 0; q < P; ++q)
-        if ((rc = launch_recount(b->probs[q], b->B.p + b->lat_off[q], b->counts.p + b->cnt_off[q],
-                                 L, st)))
-          return rc;
+      if ((rc = batch_recount(b))) return rc;
     } else {
       if ((rc = launch_merge_batch(b->merges.p + 2 * P, P, L, L, b->maxN, st))) return rc;
     }
@@ -1529,6 +1591,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (e == hipSuccess) e = b->clk.alloc((size_t)1 + n_probs);
   if (e == hipSuccess) e = b->margs.alloc((size_t)2 * n_probs);
   if (e == hipSuccess) e = b->merges.alloc((size_t)3 * n_probs);
+  if (e == hipSuccess) e = b->pops.alloc((size_t)n_probs);
   if (e == hipSuccess) e = b->items.alloc(items.size());
   if (e == hipSuccess) e = b->tie.alloc((size_t)n_probs);
   if (e == hipSuccess) e = hipMemset(b->tie.p, 0, sizeof(uint32_t) * n_probs);
@@ -1575,10 +1638,20 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
     md[(size_t)n_probs + q] = HeaMergeDesc{A, fA, C, fC, perm, tie, B, fB, L, L, L, N};
     md[(size_t)2 * n_probs + q] = HeaMergeDesc{B, fB, B, fB, perm, tie, A, fA, L, 0, L, N};
   }
+  std::vector<HeaPopDesc> pd(n_probs);
+  for (int q = 0; q < n_probs; ++q)
+    pd[q] = HeaPopDesc{probs[q]->d_const.p, probs[q]->d_nbr.p, b->B.p + b->lat_off[q],
+                       b->counts.p + b->cnt_off[q], b->fB.p + (size_t)q * L, seed + (uint64_t)q};
+  {
+    const size_t lds = recount_lds_bytes(maxN);
+    b->recount_lds = (int64_t)lds <= probs[0]->max_lds ? lds : 0;
+  }
   e = hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * ma.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(b->merges.p, md.data(), sizeof(HeaMergeDesc) * md.size(),
                   hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->pops.p, pd.data(), sizeof(HeaPopDesc) * pd.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds);
