@@ -79,7 +79,7 @@ def parse():
     ap.add_argument("--budget-s", type=float, default=60.0,
                     help="time-to-fitness workloads: wall-clock budget per mode")
     ap.add_argument("--thr", type=float, default=1e-3)
-    ap.add_argument("--guide-candidates", type=int, default=8,
+    ap.add_argument("--guide-candidates", type=int, default=12,
                     help="time-to-fitness workloads: candidate pairs per guided proposal")
     ap.add_argument("--guide-accepts", type=int, default=8,
                     help="time-to-fitness workloads: guided accepts per round at most")

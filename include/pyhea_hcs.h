@@ -174,7 +174,11 @@ int hea_session_tile(hea_session s, int32_t tile);
  * current lattice) up to date; the predicted first-order change of swapping
  * i (species a) with j (species b) is u[i][b] + u[j][a].  Each of the
  * parallel_task proposals of a round is the Gumbel-max of
- * -(u[i][b] + u[j][a]) / (tau * scale) over `candidates` hashed random pairs.
+ * -(u[i][b] + u[j][a]) / (tau * scale) over `candidates` hashed random pairs:
+ * the slot draws two blocks of 16 consecutive sites and each candidate one
+ * site in each (so a slot's candidates read two 128-byte lines of records;
+ * the environment variable PYHEA_AMD_GUIDE_BLOCK=b, read when a problem is
+ * created, sets blocks of 2^b sites, 0 draws both sites over the lattice).
  * The round's best proposal is accepted by the exact rule of
  * accelerate.cpp:663-689; then up to accepts - 1 more come from the same
  * round: the proposals that touch none of the swapped sites or their

@@ -455,6 +455,8 @@ struct hea_problem_s {
   HeaConst hc;
   bool force_hbm_species = false;  // PYHEA_AMD_SPECIES_IN_HBM=1 (tests)
   bool walk_gather = false;        // PYHEA_AMD_WALK_GATHER=1: no neighbour records in the walk
+  int guide_blk = 4;               // guided candidates from blocks of 2^b sites (16; 0: uniform);
+                                   // PYHEA_AMD_GUIDE_BLOCK=b sets b
   int64_t max_lds = 0;
   DevBuf<HeaConst> d_const;
   DevBuf<int32_t> d_nbr;
@@ -649,6 +651,7 @@ void fill_problem_args(hea_problem p, HeaMcArgs& a) {
     a.delta[s] = c.delta[s];
     a.weight[s] = c.weight[s];
   }
+  a.g_blk = p->guide_blk;
 }
 
 int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, int64_t* stats,
@@ -676,6 +679,7 @@ int launch_mc(hea_problem p, uint8_t* lat, double* fit, const int32_t* counts, i
   a.g_cand = gd.cand;
   a.g_tau = gd.tau;
   a.g_multi = gd.multi;
+  a.g_blk = p->guide_blk;
   a.dbg_pairs = gd.dbg_pairs;
   a.dbg_pred = gd.dbg_pred;
   a.rec = rec;
@@ -797,6 +801,8 @@ int hea_problem_create(const hea_problem_desc* d, int device, hea_problem* out) 
     p->force_hbm_species = v && v[0] == '1';
     const char* w = getenv("PYHEA_AMD_WALK_GATHER");
     p->walk_gather = w && w[0] == '1';
+    const char* g = getenv("PYHEA_AMD_GUIDE_BLOCK");
+    if (g && g[0]) p->guide_blk = std::max(0, std::min(6, atoi(g)));
   }
   HeaConst& c = p->hc;
   memset(&c, 0, sizeof(c));

@@ -63,6 +63,8 @@ struct HeaMcArgs {
   int32_t g_cand;               // candidate pairs per proposal
   double g_tau;                 // softmax temperature in units of the gain scale
   int32_t g_multi;              // accepts per round at most
+  int32_t g_blk;                // candidate sites: 0 uniform, b > 0 from two blocks of
+                                // 2^b consecutive sites per proposal slot
   // test capture of the first candidate phase (nullptr: none): when set, the
   // search records every (slot, candidate) pair [L][task*g_cand][2] and its
   // predicted change of F^2 [L][task*g_cand] (NaN for same-species pairs),

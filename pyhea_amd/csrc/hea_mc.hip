@@ -405,13 +405,28 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
 // two sites, drawn by multiply-high from independent hashes, and a 32-bit
 // word for the Gumbel noise.  This stream feeds only the guided proposal
 // choice; the exact acceptance never depends on it.
-__device__ __forceinline__ void guided_pair(uint32_t gk, uint32_t ctr, uint32_t n, uint32_t* s1,
-                                            uint32_t* s2, uint32_t* h3) {
+// Guided candidate at stream counter ctr of the proposal slot whose first
+// candidate has counter sctr: two sites (s1, s2) of n and the Gumbel draw h3.
+// blk == 0: both sites uniform over the lattice.  blk > 0: the slot draws
+// two blocks of 2^blk consecutive sites once (from sctr) and each candidate
+// one site in each block, so a slot's candidates read their neighbour
+// records from two blocks of 8 << blk bytes.
+__device__ __forceinline__ void guided_pair(uint32_t gk, uint32_t ctr, uint32_t sctr, uint32_t n,
+                                            int blk, uint32_t* s1, uint32_t* s2, uint32_t* h3) {
   const uint32_t h0 = hea_mix32(gk ^ (ctr * 0x9e3779b9u));
   const uint32_t h1 = hea_mix32(h0 ^ 0x85ebca6bu);
   const uint32_t h2 = hea_mix32(h1 ^ 0xc2b2ae35u);
-  *s1 = __umulhi(h0, n);
-  *s2 = __umulhi(h1, n);
+  if (blk == 0) {
+    *s1 = __umulhi(h0, n);
+    *s2 = __umulhi(h1, n);
+  } else {
+    const uint32_t k0 = hea_mix32(gk ^ (sctr * 0x9e3779b9u) ^ 0x27d4eb2fu);
+    const uint32_t k1 = hea_mix32(k0 ^ 0x165667b1u);
+    const uint32_t bs = 1u << blk, nb = (n + bs - 1) >> blk;
+    const uint32_t x = __umulhi(k0, nb) << blk, y = __umulhi(k1, nb) << blk;
+    *s1 = x + __umulhi(h0, min(n - x, bs));
+    *s2 = y + __umulhi(h1, min(n - y, bs));
+  }
   *h3 = h2;
 }
 
@@ -1060,7 +1075,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
       if (A.dbg_pred) {  // test capture of the first candidate phase (see HeaMcArgs)
         for (int w = tid; w < items; w += nthr) {
           uint32_t s1, s2, h3;
-          guided_pair(gk, pos + (uint32_t)w, (uint32_t)N, &s1, &s2, &h3);
+          guided_pair(gk, pos + (uint32_t)w, pos + (uint32_t)(w - w % g), (uint32_t)N, A.g_blk,
+                      &s1, &s2, &h3);
           const int a = sp[s1], b = sp[s2];
           const size_t o = (size_t)l * items + w;
           A.dbg_pred[o] = a != b ? guided_gain<KMAX>(T, hl, K, a, b, hist[s1], hist[s2])
@@ -1078,7 +1094,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int w = w0 + q * nthr;
-          guided_pair(gk, pos + (uint32_t)w, (uint32_t)N, &s1[q], &s2[q], &h3[q]);
+          guided_pair(gk, pos + (uint32_t)w, pos + (uint32_t)(w - w % g), (uint32_t)N, A.g_blk,
+                      &s1[q], &s2[q], &h3[q]);
           sa[q] = sp[s1[q]];
           sb[q] = sp[s2[q]];
           live[q] = w < items && sa[q] != sb[q];
@@ -1114,7 +1131,8 @@ __device__ __forceinline__ void mc_lattice(const HeaMcArgs& A, int l, unsigned c
         if (key != 0ull) {
           const int c = (int)(0xffffffffu - (uint32_t)key);
           uint32_t s1, s2, h3;
-          guided_pair(gk, pos + (uint32_t)(j * g + c), (uint32_t)N, &s1, &s2, &h3);
+          guided_pair(gk, pos + (uint32_t)(j * g + c), pos + (uint32_t)(j * g), (uint32_t)N, A.g_blk,
+                      &s1, &s2, &h3);
           bp = make_int2((int)s1, (int)s2);
         }
         T.prop[j] = bp;

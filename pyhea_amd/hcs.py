@@ -150,7 +150,7 @@ class Session:
         """Proposals evaluated per pass (launch shape only)."""
         N.check(N.lib().hea_session_tile(self._h, int(tp)))
 
-    def guide(self, candidates: int = 8, tau: float = 0.1, accepts: int = 1) -> None:
+    def guide(self, candidates: int = 12, tau: float = 0.1, accepts: int = 1) -> None:
         """Switch both local searches to gradient-guided proposals; up to
         `accepts` non-interacting swaps are accepted per round."""
         N.check(N.lib().hea_session_guide(self._h, candidates, tau, accepts))
@@ -212,7 +212,7 @@ class Batch:
                                          converge_depth, thr, seed, C.byref(h)))
         self._h = h
 
-    def guide(self, candidates: int = 8, tau: float = 0.1, accepts: int = 1) -> None:
+    def guide(self, candidates: int = 12, tau: float = 0.1, accepts: int = 1) -> None:
         N.check(N.lib().hea_batch_guide(self._h, candidates, tau, accepts))
 
     def tile(self, tp: int) -> None:
