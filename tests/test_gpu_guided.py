@@ -50,6 +50,26 @@ def test_guided_exact_acceptance(gpu, typ, cell, K, accepts):
     assert np.array_equal(lat, lat2) and np.array_equal(fit, fit2)
 
 
+def test_guided_uniform_candidates(gpu, monkeypatch):
+    """PYHEA_AMD_GUIDE_BLOCK=0 draws both candidate sites over the whole
+    lattice: the captured candidates of a slot then spread over several
+    blocks, and the search it steers keeps the exact acceptance."""
+    spec = _spec("BCC", 5, 5)
+    monkeypatch.setenv("PYHEA_AMD_GUIDE_BLOCK", "0")
+    prob = Problem.from_spec(spec)
+    monkeypatch.delenv("PYHEA_AMD_GUIDE_BLOCK")
+    o = Oracle.from_spec(spec)
+    lat0, _ = prob.generate(2, seed=17)
+    pairs, _ = prob.guided_scores(lat0, 64, 8, seed=23)
+    blk = pairs[..., 0].reshape(2, 64, 8) // 16
+    assert (blk != blk[..., :1]).any(axis=-1).mean() > 0.9
+    lat, fit, st = _run(prob, True, accepts=4)
+    assert np.allclose(o.fitness(lat), fit, rtol=1e-11, atol=0)
+    assert st["accepts"] > 0
+    lb, fb, _ = _run(Problem.from_spec(spec), True, accepts=4)  # default: blocks
+    assert not np.array_equal(lb, lat)
+
+
 def test_guided_batch_matches_guided_sessions(gpu):
     specs = [_spec("FCC", 3, 4, 21), _spec("BCC", 4, 5, 22)]
     probs = [Problem.from_spec(s) for s in specs]
@@ -124,6 +144,12 @@ def test_guided_scores_are_relaxation_site_gains(gpu, typ, cell, K):
     u = site_gain(prob, torch.as_tensor(lat, device="cuda"), dn).cpu().numpy().astype(np.float64)
     i, j = pairs[..., 0], pairs[..., 1]
     assert i.min() >= 0 and max(i.max(), j.max()) < spec.n_sites
+    # a slot's 8 candidates take their first sites from one block of 16
+    # consecutive sites and their second sites from another (the lattice's
+    # last block may be shorter: 250 and 108 sites here)
+    for x in (i, j):
+        blk = x.reshape(x.shape[0], 64, 8) // 16
+        assert (blk == blk[..., :1]).all()
     ll = np.arange(lat.shape[0])[:, None]
     a, b = lat[ll, i], lat[ll, j]
     live = a != b
