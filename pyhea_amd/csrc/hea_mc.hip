@@ -401,16 +401,15 @@ __device__ void refresh_entry(const HeaMcArgs& A, const Lds<KMAX>& T, int s, int
   if constexpr (GUIDED) atomicMax(&T.sh->gmax, __float_as_uint(r.hz));
 }
 
-// Guided mode: candidate item `ctr` of a lattice's candidate stream (key gk):
-// two sites, drawn by multiply-high from independent hashes, and a 32-bit
-// word for the Gumbel noise.  This stream feeds only the guided proposal
-// choice; the exact acceptance never depends on it.
-// Guided candidate at stream counter ctr of the proposal slot whose first
-// candidate has counter sctr: two sites (s1, s2) of n and the Gumbel draw h3.
-// blk == 0: both sites uniform over the lattice.  blk > 0: the slot draws
-// two blocks of 2^blk consecutive sites once (from sctr) and each candidate
-// one site in each block, so a slot's candidates read their neighbour
-// records from two blocks of 8 << blk bytes.
+// Guided mode: candidate item `ctr` of a lattice's candidate stream (key
+// gk), in the proposal slot whose first item is `sctr`: two sites (s1, s2)
+// of n, drawn by multiply-high from independent hashes, and a 32-bit word h3
+// for the Gumbel noise.  blk == 0: both sites uniform over the lattice.
+// blk > 0: the slot draws two blocks of 2^blk consecutive sites once (from
+// sctr; the last block of the lattice may be shorter) and each candidate one
+// site in each block, so a slot's candidates read their neighbour records
+// from two spans of 8 << blk bytes.  This stream feeds only the guided
+// proposal choice; the exact acceptance never depends on it.
 __device__ __forceinline__ void guided_pair(uint32_t gk, uint32_t ctr, uint32_t sctr, uint32_t n,
                                             int blk, uint32_t* s1, uint32_t* s2, uint32_t* h3) {
   const uint32_t h0 = hea_mix32(gk ^ (ctr * 0x9e3779b9u));
