@@ -90,7 +90,7 @@ __global__ void k_permute(const HeaConst* __restrict__ pc, uint8_t* __restrict__
                           int slot0) {
   const int l = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  permute_sites(*pc, out, seed, *iter_ptr, phase, slot0 + l, l, i, 1 << 30);
+  permute_sites(*pc, out, seed, *iter_ptr, phase, slot0 + l, l, i, gridDim.x * blockDim.x);
 }
 
 __global__ void k_zero_i32(int32_t* __restrict__ p, size_t n) {
