@@ -722,6 +722,21 @@ __device__ __forceinline__ float guided_gain(const Lds<KMAX>& T, const HeaHistLa
 struct SwapNbrs {
   int nb[3];
 };
+
+// Shell of item q of a site's concatenated lists (q < sum Z; absent shells
+// have Z = 0), and q made the index within that shell.  Compares and
+// selects of the shell sizes: indexing them by a lane-varying shell would
+// load from memory.
+__device__ __forceinline__ int item_shell(const HeaMcArgs& A, int& q) {
+  const int z0 = A.Z[0], z01 = z0 + A.Z[1];
+  const int s = (q >= z0) + (q >= z01);
+  q -= s == 0 ? 0 : s == 1 ? z0 : z01;
+  return s;
+}
+__device__ __forceinline__ int shell_off(const HeaMcArgs& A, int s) {
+  return s == 0 ? A.off[0] : s == 1 ? A.off[1] : A.off[2];
+}
+
 __device__ __forceinline__ SwapNbrs swap_nbrs_load(const HeaMcArgs& A, int2 pr, int first,
                                                    int stride) {
   int zs = 0;
@@ -733,9 +748,9 @@ __device__ __forceinline__ SwapNbrs swap_nbrs_load(const HeaMcArgs& A, int2 pr, 
     r.nb[u] = -1;
     if (t < 2 * zs) {
       const int site = t < zs ? pr.x : pr.y;
-      int q = t < zs ? t : t - zs, s = 0;
-      while (q >= A.Z[s]) q -= A.Z[s++];
-      r.nb[u] = A.nbr[(size_t)site * A.stride + A.off[s] + q];
+      int q = t < zs ? t : t - zs;
+      const int s = item_shell(A, q);
+      r.nb[u] = A.nbr[(size_t)site * A.stride + shell_off(A, s) + q];
     }
   }
   return r;
@@ -753,9 +768,9 @@ __device__ __forceinline__ SwapNbrs swap_nbrs_rows(const HeaMcArgs& A, const int
     const int t = first + u * stride;
     r.nb[u] = -1;
     if (t < 2 * zs) {
-      int q = t < zs ? t : t - zs, s = 0;
-      while (q >= A.Z[s]) q -= A.Z[s++];
-      r.nb[u] = rows[(t < zs ? 0 : A.stride) + A.off[s] + q];
+      int q = t < zs ? t : t - zs;
+      const int s = item_shell(A, q);
+      r.nb[u] = rows[(t < zs ? 0 : A.stride) + shell_off(A, s) + q];
     }
   }
   return r;
@@ -779,8 +794,8 @@ __device__ __forceinline__ void guided_apply_swap(const HeaMcArgs& A, const Lds<
   for (int u = 0; u < 3; ++u) {
     if (n.nb[u] < 0) continue;
     const int t = first + u * stride;
-    int q = t < zs ? t : t - zs, s = 0;
-    while (q >= A.Z[s]) q -= A.Z[s++];
+    int q = t < zs ? t : t - zs;
+    const int s = item_shell(A, q);
     // the last species has no field (its count is implied)
     // the plain kernels select the field without a variable index (which puts
     // the layout in scratch memory); for the guided ones the indexed form
