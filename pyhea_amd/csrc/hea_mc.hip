@@ -796,11 +796,10 @@ __device__ __forceinline__ void guided_apply_swap(const HeaMcArgs& A, const Lds<
     const int t = first + u * stride;
     int q = t < zs ? t : t - zs;
     const int s = item_shell(A, q);
-    // the last species has no field (its count is implied)
-    // the plain kernels select the field without a variable index (which puts
-    // the layout in scratch memory); for the guided ones the indexed form
-    // measured 1.3 % faster
-    const int os = MAP ? hl.o[s] : s == 0 ? hl.o[0] : s == 1 ? hl.o[1] : hl.o[2];
+    // the last species has no field (its count is implied); the field is
+    // selected without a variable index (which would put the layout in
+    // scratch memory)
+    const int os = s == 0 ? hl.o[0] : s == 1 ? hl.o[1] : hl.o[2];
     const unsigned long long fa = (a < K - 1) ? 1ull << (a * hl.W + os) : 0ull;
     const unsigned long long fb = (b < K - 1) ? 1ull << (b * hl.W + os) : 0ull;
     const unsigned long long dv = t < zs ? fb - fa : fa - fb;  // modulo 2^64
