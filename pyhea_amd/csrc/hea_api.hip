@@ -1407,9 +1407,11 @@ int hea_hcs_run(hea_problem p, int32_t L, int32_t total_iter, int32_t task, int3
 // Many independent problems searched together.  Problem p keeps its own
 // population (A: current, B: searched, C: sorted scratch) and draws from seed
 // `seed + p`, so its result equals hea_hcs_run(problem p, seed + p).  The
-// cheap phases are launched per problem; both local searches run as one
-// persistent launch over all (problem, lattice) items, longest first.  One
-// global iteration is captured as one HIP graph.
+// cheap phases are one launch per step for all problems; both local
+// searches run as one persistent launch over all (problem, lattice) items,
+// longest first (a batch of one problem launches the per-lattice kernel
+// instead, its arguments in the kernarg segment and its table width exact).
+// One global iteration is captured as one HIP graph.
 
 struct hea_batch_s {
   std::vector<hea_problem> probs;
@@ -1431,6 +1433,8 @@ struct hea_batch_s {
   DevBuf<uint32_t> tie;            // [problems] tie flags of the population sorts
   DevBuf<unsigned long long> clk;  // [0] start, [1 + p] first fitness <= thr of problem p
   DevBuf<HeaMcArgs> margs;         // launch arguments [2 phases][problems] for population B
+  std::vector<HeaMcArgs> hmargs;   // host copy of margs: a batch of one problem launches
+                                   // k_local_mc with them as kernel arguments
   DevBuf<HeaMergeDesc> merges;     // [3][problems]: B -> C and A ++ C -> B after local
                                    // search I, B -> A after local search II
   DevBuf<HeaPopDesc> pops;         // [problems]: population B, its counts and fitness
@@ -1488,8 +1492,11 @@ static int batch_enqueue_iteration(hea_batch b, const IterHooks* hk = nullptr) {
                                hipMemcpyDeviceToDevice, st));
       HEA_HIP(hipEventRecord(hk->ev[2 * h], st));
     }
-    HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->guide.hist != nullptr, b->grid,
-                                      b->threads, b->lds, st));
+    if (P == 1)  // one problem: nothing to balance, its arguments go in the kernarg segment
+      HEA_HIP(hea_launch_local_mc(b->hmargs[h], b->probs[0]->hc.K, b->sp_lds, false, b->lds, st));
+    else
+      HEA_HIP(hea_launch_local_mc_batch(ba, b->kmax, b->sp_lds, b->guide.hist != nullptr, b->grid,
+                                        b->threads, b->lds, st));
     if (hk) {
       HEA_HIP(hipEventRecord(hk->ev[2 * h + 1], st));
       HEA_HIP(hipMemcpyAsync(hk->snap + (h + 1) * sn, b->stats.p, sizeof(int64_t) * sn,
@@ -1653,6 +1660,7 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
     b->recount_lds = (int64_t)lds <= probs[0]->max_lds ? lds : 0;
   }
   e = hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * ma.size(), hipMemcpyHostToDevice);
+  b->hmargs = ma;
   if (e == hipSuccess)
     e = hipMemcpy(b->merges.p, md.data(), sizeof(HeaMergeDesc) * md.size(),
                   hipMemcpyHostToDevice);
@@ -1661,6 +1669,8 @@ int hea_batch_create(const hea_problem* probs, int32_t n_probs, int32_t L, int32
   if (e == hipSuccess)
     e = hipMemcpy(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds);
+  if (e == hipSuccess && n_probs == 1)
+    e = hea_prepare_local_mc(probs[0]->hc.K, b->sp_lds, false, b->lds);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete b;
@@ -1696,6 +1706,7 @@ static int batch_patch_args(hea_batch b, const std::function<void(HeaMcArgs&, in
   HEA_HIP(hipMemcpy(ma.data(), b->margs.p, sizeof(HeaMcArgs) * n, hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; ++i) patch(ma[i], (int)(i % b->probs.size()));
   HEA_HIP(hipMemcpy(b->margs.p, ma.data(), sizeof(HeaMcArgs) * n, hipMemcpyHostToDevice));
+  b->hmargs = ma;
   return HEA_OK;
 }
 
@@ -1742,6 +1753,8 @@ int hea_batch_tile(hea_batch b, int32_t tp) {
   }
   HEA_HIP(hipSetDevice(b->probs[0]->device));
   HEA_HIP(hea_prepare_local_mc_batch(b->kmax, b->sp_lds, b->lds));
+  if (b->probs.size() == 1)
+    HEA_HIP(hea_prepare_local_mc(b->probs[0]->hc.K, b->sp_lds, false, b->lds));
   int rc = batch_patch_args(b, [tp, b](HeaMcArgs& a, int q) {
     a.tp = tp;
     const int lanes = hea_mc_split(a, false);

@@ -105,3 +105,38 @@ def test_uniform_species_batch_equals_plain(gpu, K):
         assert np.array_equal(_bits(fb), _bits(fp)), q
         assert sb["evals"] == sp["evals"] and sb["accepts"] == sp["accepts"], q
     b.close()
+
+
+@pytest.mark.parametrize("K", [3, 5, 7])
+@pytest.mark.parametrize("guided", [False, True])
+@pytest.mark.parametrize("hbm", [False, True])
+def test_single_problem_batch_equals_session(gpu, monkeypatch, K, guided, hbm):
+    """A batch of one problem launches the per-lattice kernel (its arguments
+    in the kernarg segment, exact table width where K allows) instead of the
+    persistent one; it must still equal the session of that problem bit for
+    bit, plain and guided, with species in LDS and in HBM (7 species: the
+    width-8 kernel, plain only)."""
+    if guided and K == 7:
+        pytest.skip("guided mode needs K <= 6")
+    fr = np.random.default_rng(K).dirichlet([2.0] * K) + 0.05
+    spec = make_problem("FCC", 3, fr, shells="true", target_seed=60 + K)
+    monkeypatch.setenv("PYHEA_AMD_SPECIES_IN_HBM", "1" if hbm else "0")
+    p = Problem.from_spec(spec)
+    monkeypatch.delenv("PYHEA_AMD_SPECIES_IN_HBM")
+    L, I, T, D, seed = 6, 2, 48, 3, 910
+    b = Batch([p], L, T, D, seed=seed)
+    s = p.session(L, T, D, seed=seed)
+    if guided:
+        b.guide(candidates=12, tau=0.1, accepts=4)
+        s.guide(candidates=12, tau=0.1, accepts=4)
+    b.init()
+    b.step(I)
+    s.init()
+    s.step(I)
+    lb, fb, sb = b.read(0)
+    ls, fs, ss = s.read()
+    assert np.array_equal(lb, ls)
+    assert np.array_equal(_bits(fb), _bits(fs))
+    assert sb["evals"] == ss["evals"] and sb["accepts"] == ss["accepts"] and sb["accepts"] > 0
+    b.close()
+    s.close()
