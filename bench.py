@@ -319,17 +319,18 @@ def time_to_fitness(a):
         "config": {"workload": a.workload, "lattice": f"{spec.latt_typ} {spec.cell_dim}^3",
                    "n_sites": spec.n_sites, "species": spec.counts, "shells": spec.shell_sizes,
                    "solutions": L, "parallel_task": T, "converge_depth": D, "seed": seed,
-                   "mode": "guided + batched (persistent launch, HIP graph)"},
+                   "mode": "guided + batched (HIP graph; a batch of one problem runs the "
+                           "per-lattice kernel)"},
         "time_to_fitness": ttf,
         "guided_batched": guided,
         "plain": plain,
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": (profiled_traffic("guided-fcc30", "k_local_mc_batch")
+                     "traffic": (profiled_traffic("guided-fcc30", "k_local_mc")
                                  if a.workload == "ttf-fcc30" else
                                  profiled_traffic("", key="guided_traffic_bytes_per_launch",
                                                   fname="k_local_mc_fcc50.md")),
-                     "kernel": "k_local_mc_batch (guided)",
+                     "kernel": "k_local_mc (guided, batch of one problem)",
                      "launches": len(launches),
                      "avg_launch_ms": round(float(ms.mean()), 3),
                      "bytes_per_launch": int(byts // len(launches)),
